@@ -1,0 +1,177 @@
+/*
+ * mfsv.h -- C ABI of libmfsv_hip.so, the MI355X (gfx950) implementation of
+ * mfcm's explicit 2-D Saint-Venant time step on the Cartesian mesh.
+ *
+ * Plain C: no C++ types, no exceptions and no torch types cross this
+ * boundary.  Every entry point returns MFSV_OK (0) or a negative error code;
+ * mfsv_last_error(ctx) returns the message of the last failure on a context.
+ * A context owns one HIP stream and its device allocations and is externally
+ * synchronised (one host thread at a time per context).  Several contexts may
+ * share a device; the batch API advances many catchments per launch.
+ *
+ * The reference has no C ABI: its boundary is C++ templates and virtuals.  The
+ * comment on each entry point names the reference interface it replaces
+ * (file:line relative to the reference's src/).
+ */
+#ifndef MFSV_H
+#define MFSV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFSV_OK 0
+#define MFSV_ERR_ARG (-1)         /* bad argument (null, size, range)       */
+#define MFSV_ERR_HIP (-2)         /* a HIP runtime call failed              */
+#define MFSV_ERR_STATE (-3)       /* call not valid in the current state    */
+#define MFSV_ERR_NOMEM (-4)       /* device allocation failed / over budget */
+#define MFSV_ERR_UNSUPPORTED (-5) /* option not available in this build     */
+
+/* Value precision of a context.  Time, the controller and all mass/volume
+ * accumulation are FP64 in both. */
+#define MFSV_F64 0 /* plain method: the reference's arithmetic in FP64 */
+#define MFSV_F32 1 /* precision form: FP32 depth-based arithmetic      */
+
+/* Extension flags (0 = the reference's semantics). */
+#define MFSV_E1_RAIN_GUARD 0x01   /* discharge dry guard h + dh*dt <= 0     */
+#define MFSV_E2_WEST_OUTFLOW 0x02 /* transmissive outflow on the x = 0 edge */
+#define MFSV_E4_STORM 0x08        /* analytic drifting Gaussian storm        */
+#define MFSV_E7_Y_ORIENT 0x40     /* y slope taken south->north             */
+
+/* Cell fields (Field/DataArray containers of the reference, one value per
+ * cell, row-major i = y*nx + x as Mesh/Cartesian2DMesh.hpp:308-309). */
+#define MFSV_FIELD_H 0         /* depth h of the current state U0         */
+#define MFSV_FIELD_U 1         /* x velocity u                            */
+#define MFSV_FIELD_V 2         /* y velocity v                            */
+#define MFSV_FIELD_ZB 3        /* bed elevation (NaN = coded out)         */
+#define MFSV_FIELD_MANNING_N 4 /* per-cell Manning n (shallow = deep)     */
+#define MFSV_FIELD_QBDY0 5     /* discharge endpoint at step start        */
+#define MFSV_FIELD_QBDY1 6     /* discharge endpoint at step end          */
+#define MFSV_FIELD_HBDY0 7     /* head endpoint at step start (NaN = off) */
+#define MFSV_FIELD_HBDY1 8     /* head endpoint at step end (NaN = off)   */
+#define MFSV_FIELD_DHDT 9      /* k of the last mfsv_dudt call, h part    */
+#define MFSV_FIELD_DUDT 10
+#define MFSV_FIELD_DVDT 11
+#define MFSV_FIELD_BRANCH_X 12 /* x-face branch codes, (nx+1)*ny values   */
+#define MFSV_FIELD_BRANCH_Y 13 /* y-face branch codes, nx*(ny+1) values   */
+#define MFSV_FIELD_COUNT 14
+
+typedef struct mfsv_ctx mfsv_ctx;
+
+/* Mesh geometry as Cartesian2DMesh (Mesh/Cartesian2DMesh.cpp:25-61) with
+ * zero rotation: dx, dy are the cell sizes, origin the lower-left corner. */
+typedef struct mfsv_grid {
+  int64_t nx, ny;
+  double dx, dy;
+  double origin_x, origin_y;
+} mfsv_grid;
+
+typedef struct mfsv_params {
+  int32_t flags;    /* MFSV_E* bits                                     */
+  double cn_target; /* controller target; 1.0 in RungeKutta.cpp:154      */
+  /* Manning source (ManningRoughnessSourceTerm.cpp:49-109); a per-cell n
+   * uploaded as MFSV_FIELD_MANNING_N replaces n_shallow and n_deep. */
+  int32_t manning;
+  double n_shallow, n_deep, d_shallow, d_deep;
+  /* Discharge boundary blocks (DischargeBoundarySourceTerm): 0 = none,
+   * 1 = global selection with scalar per-step endpoints (mfsv_set_forcing
+   * or the forcing table), 2 = per-cell endpoint fields QBDY0/QBDY1. */
+  int32_t discharge;
+  /* Head boundary (HeadBoundarySourceTerm): 0 = none, 1 = the columns
+   * [head_x0, head_x1) with scalar endpoints, 2 = per-cell HBDY0/HBDY1. */
+  int32_t head;
+  int64_t head_x0, head_x1;
+  /* E4 storm: rate = peak*exp(-r^2/(2 sigma^2)) around (cx0+vx t, cy0+vy t). */
+  double storm_peak, storm_sigma, storm_cx0, storm_cy0, storm_vx, storm_vy;
+} mfsv_params;
+
+typedef struct mfsv_stats {
+  double time;         /* global time of U0                                */
+  double t_local;      /* time within the current outer step               */
+  double dt;           /* next timestep                                    */
+  double cn;           /* control number of the last timestep              */
+  double volume;       /* FP64 sum of h*dx*dy of U0                        */
+  double source_volume;/* FP64 integral of the net boundary + source rate  */
+  double volume0;      /* volume when the run started                      */
+  int64_t step;        /* outer steps completed                            */
+  int64_t nsteps;      /* outer steps in the run                           */
+  int64_t accepted;    /* accepted timesteps                               */
+  int64_t rejected;    /* rejected (repeated) timesteps                    */
+  int32_t done;        /* 1 once every outer step has completed            */
+  int32_t current;     /* which state buffer holds U0                      */
+} mfsv_stats;
+
+/* ---- lifetime (replaces SaintVenantSolver's constructor, Solver.cpp:36-71,
+ * and the SYCL queue selection of sycl.cpp:22-126) ---------------------- */
+const char* mfsv_version(void);
+int mfsv_device_count(int* count);
+int mfsv_create(const mfsv_grid* grid, int precision, int device, mfsv_ctx** out);
+int mfsv_destroy(mfsv_ctx* ctx);
+const char* mfsv_last_error(const mfsv_ctx* ctx);
+int mfsv_synchronize(mfsv_ctx* ctx);
+/* Bytes of device memory the context holds. */
+int mfsv_device_bytes(const mfsv_ctx* ctx, int64_t* bytes);
+
+/* ---- configuration ------------------------------------------------------ */
+int mfsv_set_params(mfsv_ctx* ctx, const mfsv_params* p);
+/* Butcher table, (nstages+1)^2 doubles row-major laid out as the rows of
+ * RungeKuttaCoefficientSet (TemporalScheme/RungeKutta.hpp:59-123):
+ * row i < n = {c_i, a_i1..}, row n = {0, b_1..b_n}.  1 <= nstages <= 4. */
+int mfsv_set_scheme(mfsv_ctx* ctx, int nstages, const double* tableau);
+/* TimeParameters (Config/TimeParameters.cpp:22-58), all in seconds. */
+int mfsv_set_time(mfsv_ctx* ctx, double start, double end, double step_duration,
+                  double initial_dt);
+/* Start of an outer step at global time time_now (start_new_step,
+ * Solver.cpp:119-129): the scalar discharge/head endpoints the boundaries
+ * computed for [time_now, time_now + step) (NaN head = inactive). */
+int mfsv_start_new_step(mfsv_ctx* ctx, double time_now, double q0, double q1, double hb0,
+                        double hb1);
+/* Per-outer-step endpoint table (4 doubles per step: q0, q1, hb0, hb1) that
+ * the device controller loads at each step start. */
+int mfsv_set_forcing_table(mfsv_ctx* ctx, int64_t nsteps, const double* table);
+
+/* ---- fields (replaces DataArray<T>::move_to_device/move_to_host,
+ * DataArray/DataArray.cpp:84-116).  Host arrays are FP64 in both
+ * precisions; an F32 context rounds on upload. ---------------------------- */
+int mfsv_upload(mfsv_ctx* ctx, int field, const double* host, int64_t count);
+int mfsv_download(mfsv_ctx* ctx, int field, double* host, int64_t count);
+
+/* ---- the time step ------------------------------------------------------- */
+/* Temporal derivative of the current state U0 at stage time t_sub with
+ * timestep dt into the DHDT/DUDT/DVDT fields: the slope passes, face fluxes,
+ * cell pass, Manning, discharge and head terms of
+ * SaintVenantSolver::update_dUdt (SaintVenant/Solver.cpp:77-114).  With
+ * want_branches != 0 also fills BRANCH_X/BRANCH_Y (FluxKernel.cpp:291-293). */
+int mfsv_dudt(mfsv_ctx* ctx, double t_sub, double dt, int want_branches);
+/* One whole timestep from U0 at local time t_local with timestep dt: every
+ * RK stage (RungeKutta.cpp:143-153, update_k + update_y) fused per stage, and
+ * the control number of the result (State.cpp:199-229) returned in *cn.
+ * Synchronises the stream.  The result is held until mfsv_accept. */
+int mfsv_timestep(mfsv_ctx* ctx, double t_local, double dt, double* cn);
+/* accept_timestep (RungeKutta.hpp:428-431): the last result becomes U0. */
+int mfsv_accept(mfsv_ctx* ctx);
+/* FP64 volume sum(h)*dx*dy of U0 (Output/Measure.hpp:137-153). */
+int mfsv_volume(mfsv_ctx* ctx, double* volume);
+
+/* ---- device-controlled run (TemporalScheme::solve/step,
+ * TemporalScheme.hpp:112-149 and :244-276, with the controller of
+ * RungeKutta.cpp:154-167 evaluated on the device) ------------------------- */
+/* Reset to outer step 0 with dt = initial dt (forcing table required when a
+ * scalar discharge or head boundary is configured). */
+int mfsv_run_begin(mfsv_ctx* ctx);
+/* Attempt up to max_timesteps timesteps (fewer if the run finishes);
+ * graph != 0 replays hipGraph-captured timesteps. */
+int mfsv_run(mfsv_ctx* ctx, int64_t max_timesteps, int graph);
+int mfsv_get_stats(mfsv_ctx* ctx, mfsv_stats* out);
+
+/* Timing of the last mfsv_run: milliseconds measured with HIP events on the
+ * context's stream around the launches of that call. */
+int mfsv_last_run_ms(mfsv_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFSV_H */

@@ -1,0 +1,728 @@
+// mfsv.hip -- libmfsv_hip.so: the C ABI declared in include/mfsv.h.
+//
+// A context is one catchment on one device: HIP stream, structure-of-arrays
+// device fields, the device control block and the per-tile partial sums.
+// All launches go to the context's stream; the only host synchronisations
+// are the ones an entry point documents (mfsv_timestep returns the control
+// number, mfsv_run checks the finished flag once per batch of timesteps).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mfsv.h"
+#include "sv_stage.hpp"
+
+using namespace mfsv;
+
+namespace {
+
+constexpr int kVolBlocks = 256;
+
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct mfsv_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int prec = MFSV_F64;
+  size_t vsz = 8;
+  int64_t nx = 0, ny = 0, nc = 0;
+  double dx = 1, dy = 1, ox = 0, oy = 0;
+  double zoff = 0.0;  // bed offset removed by the F32 form
+  mfsv_params P{};
+  int nstages = 2;
+  double tab[kMaxStages + 1][kMaxStages + 1] = {};
+  double start = 0, end = 0, step_dur = 60, dt0 = 1;
+  // device memory
+  Buf U[2][3];
+  Buf acc[kMaxStages][3];  // accumulators for RK rows 1..nstages-1
+  Buf zb, nfield, q0f, q1f, hb0f, hb1f, kf[3], brx, bry;
+  Buf ctrl, part_vol, part_rate, forcing, volp;
+  int64_t nforcing = 0;
+  Ctrl* hctrl = nullptr;  // pinned mirror
+  int rows = 16;
+  int64_t nstrips = 0, nrb = 0, ntiles = 0, nblocks = 0;
+  int cur = 0;
+  bool nfield_set = false;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int64_t graph_steps = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  double volume0 = 0.0;
+  std::string err;
+  int64_t bytes = 0;
+};
+
+namespace {
+
+int fail(mfsv_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return fail((c), MFSV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int alloc(mfsv_ctx* c, Buf& b, size_t bytes) {
+  if (b.p && b.bytes == bytes) return MFSV_OK;
+  if (b.p) {
+    (void)hipFree(b.p);
+    c->bytes -= (int64_t)b.bytes;
+    b.p = nullptr;
+  }
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    b.p = nullptr;
+    return fail(c, MFSV_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  b.bytes = bytes;
+  c->bytes += (int64_t)bytes;
+  return MFSV_OK;
+}
+
+void release(Buf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+void drop_graph(mfsv_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_steps = 0;
+}
+
+void geometry(mfsv_ctx* c) {
+  c->nstrips = (c->nx + kStrip - 1) / kStrip;
+  c->nrb = (c->ny + c->rows - 1) / c->rows;
+  c->ntiles = c->nstrips * c->nrb;
+  c->nblocks = (c->ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
+}
+
+int ensure_partials(mfsv_ctx* c) {
+  int rc;
+  if ((rc = alloc(c, c->part_vol, sizeof(double) * c->ntiles))) return rc;
+  if ((rc = alloc(c, c->part_rate, sizeof(double) * c->ntiles * kMaxStages))) return rc;
+  return MFSV_OK;
+}
+
+// Upload a host FP64 array as the context's value type.
+int put(mfsv_ctx* c, Buf& b, const double* host, int64_t n, double offset) {
+  int rc = alloc(c, b, c->vsz * n);
+  if (rc) return rc;
+  if (c->prec == MFSV_F64 && offset == 0.0) {
+    HIPCHK(c, hipMemcpyAsync(b.p, host, 8 * n, hipMemcpyHostToDevice, c->stream));
+  } else if (c->prec == MFSV_F64) {
+    std::vector<double> t(n);
+    for (int64_t i = 0; i < n; ++i) t[i] = host[i] - offset;
+    HIPCHK(c, hipMemcpyAsync(b.p, t.data(), 8 * n, hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> t(n);
+    for (int64_t i = 0; i < n; ++i) t[i] = (float)(host[i] - offset);
+    HIPCHK(c, hipMemcpyAsync(b.p, t.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int get(mfsv_ctx* c, const Buf& b, double* host, int64_t n, double offset, size_t esz) {
+  if (!b.p) return fail(c, MFSV_ERR_STATE, "field not allocated");
+  if (esz == 8) {
+    HIPCHK(c, hipMemcpyAsync(host, b.p, 8 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (offset != 0.0)
+      for (int64_t i = 0; i < n; ++i) host[i] += offset;
+  } else {
+    std::vector<float> t(n);
+    HIPCHK(c, hipMemcpyAsync(t.data(), b.p, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; ++i) host[i] = (double)t[i] + offset;
+  }
+  return MFSV_OK;
+}
+
+template <typename T>
+T* tp(const Buf& b) { return static_cast<T*>(b.p); }
+
+template <typename T>
+StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches) {
+  StageArgs<T> A{};
+  A.nx = c->nx; A.ny = c->ny;
+  A.nrb = c->nrb; A.ntiles = c->ntiles; A.rows = c->rows;
+  A.stage = s; A.nstages = c->nstages;
+  A.last = (!kout && s == c->nstages - 1) ? 1 : 0;
+  A.kout = kout ? 1 : 0;
+  A.flags = c->P.flags;
+  A.manning = c->P.manning;
+  A.discharge = c->P.discharge;
+  A.head = c->P.head;
+  A.hx0 = c->P.head_x0; A.hx1 = c->P.head_x1;
+  A.dx = T(c->dx); A.dy = T(c->dy);
+  A.rdx = T(1.0 / c->dx); A.rdy = T(1.0 / c->dy);
+  A.hdx = T(0.5 * c->dx); A.hdy = T(0.5 * c->dy);
+  A.dxd = c->dx; A.dyd = c->dy; A.area = c->dx * c->dy;
+  A.ox = c->ox; A.oy = c->oy;
+  A.c_s = c->tab[s][0];
+  A.Uh[0] = tp<T>(c->U[0][0]); A.Uu[0] = tp<T>(c->U[0][1]); A.Uv[0] = tp<T>(c->U[0][2]);
+  A.Uh[1] = tp<T>(c->U[1][0]); A.Uu[1] = tp<T>(c->U[1][1]); A.Uv[1] = tp<T>(c->U[1][2]);
+  if (s > 0) {  // stage input: accumulator row s
+    A.in_h = tp<T>(c->acc[s][0]); A.in_u = tp<T>(c->acc[s][1]); A.in_v = tp<T>(c->acc[s][2]);
+  }
+  A.zb = tp<T>(c->zb);
+  A.nfield = c->nfield_set ? tp<T>(c->nfield) : nullptr;
+  A.q0f = tp<T>(c->q0f); A.q1f = tp<T>(c->q1f);
+  A.hb0f = tp<T>(c->hb0f); A.hb1f = tp<T>(c->hb1f);
+  A.n_sh = T(c->P.n_shallow); A.n_dp = T(c->P.n_deep);
+  A.d_sh = T(c->P.d_shallow); A.d_dp = T(c->P.d_deep);
+  A.storm[0] = c->P.storm_peak; A.storm[1] = c->P.storm_sigma;
+  A.storm[2] = c->P.storm_cx0; A.storm[3] = c->P.storm_cy0;
+  A.storm[4] = c->P.storm_vx; A.storm[5] = c->P.storm_vy;
+  if (kout) {
+    A.nacc = 0;
+    A.k_h = tp<T>(c->kf[0]); A.k_u = tp<T>(c->kf[1]); A.k_v = tp<T>(c->kf[2]);
+  } else {
+    // rows i = s+1 .. n receive k_s * a(i, s+1)  (RungeKutta.hpp:377-388)
+    A.nacc = c->nstages - s;
+    for (int j = 0; j < A.nacc; ++j) {
+      const int i = s + 1 + j;
+      A.acoef[j] = c->tab[i][s + 1];
+      if (i < c->nstages) {
+        A.acc_h[j] = tp<T>(c->acc[i][0]); A.acc_u[j] = tp<T>(c->acc[i][1]);
+        A.acc_v[j] = tp<T>(c->acc[i][2]);
+      }  // row n: null -> the new-state buffer
+    }
+  }
+  if (branches) { A.br_x = tp<float>(c->brx); A.br_y = tp<float>(c->bry); }
+  A.ctrl = tp<Ctrl>(c->ctrl);
+  A.part_vol = tp<double>(c->part_vol);
+  A.part_rate = tp<double>(c->part_rate) + (int64_t)s * c->ntiles;
+  return A;
+}
+
+int launch_stage(mfsv_ctx* c, int s, bool kout, bool branches) {
+  dim3 grid((unsigned)c->nblocks), block(kWave * kWavesPerBlock);
+  if (c->prec == MFSV_F64) {
+    hipLaunchKernelGGL(k_stage<double>, grid, block, 0, c->stream, stage_args<double>(c, s, kout, branches));
+  } else {
+    hipLaunchKernelGGL(k_stage<float>, grid, block, 0, c->stream, stage_args<float>(c, s, kout, branches));
+  }
+  HIPCHK(c, hipGetLastError());
+  return MFSV_OK;
+}
+
+int launch_ctrl(mfsv_ctx* c, int mode) {
+  CtrlArgs A{};
+  A.ctrl = tp<Ctrl>(c->ctrl);
+  A.part_vol = tp<double>(c->part_vol);
+  A.part_rate = tp<double>(c->part_rate);
+  A.ntiles = c->ntiles;
+  A.nstages = c->nstages;
+  A.mode = mode;
+  for (int s = 0; s < c->nstages; ++s) A.b[s] = c->tab[c->nstages][s + 1];
+  A.area = c->dx * c->dy;
+  A.forcing = c->nforcing ? tp<double>(c->forcing) : nullptr;
+  A.nforcing = c->nforcing;
+  hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(256), 0, c->stream, A);
+  HIPCHK(c, hipGetLastError());
+  return MFSV_OK;
+}
+
+int launch_timestep(mfsv_ctx* c, int mode) {
+  int rc;
+  for (int s = 0; s < c->nstages; ++s)
+    if ((rc = launch_stage(c, s, false, false))) return rc;
+  return launch_ctrl(c, mode);
+}
+
+int read_ctrl(mfsv_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->hctrl, c->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+template <typename T>
+__global__ void k_volume(const T* __restrict__ h, int64_t n, double* part) {
+  __shared__ double sh[256];
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = blockIdx.x * per;
+  const int64_t b1 = (b0 + per < n) ? b0 + per : n;
+  double s = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) s += (double)h[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum(const double* part, int n, double scale, double* out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0] * scale;
+}
+
+int compute_volume(mfsv_ctx* c, int which, double* out) {
+  int rc;
+  if ((rc = alloc(c, c->volp, sizeof(double) * (kVolBlocks + 1)))) return rc;
+  double* part = tp<double>(c->volp);
+  if (c->prec == MFSV_F64)
+    hipLaunchKernelGGL(k_volume<double>, dim3(kVolBlocks), dim3(256), 0, c->stream,
+                       tp<double>(c->U[which][0]), c->nc, part);
+  else
+    hipLaunchKernelGGL(k_volume<float>, dim3(kVolBlocks), dim3(256), 0, c->stream,
+                       tp<float>(c->U[which][0]), c->nc, part);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, c->stream, part, kVolBlocks, c->dx * c->dy,
+                     part + kVolBlocks);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, part + kVolBlocks, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int set_ctrl(mfsv_ctx* c, double t_local, double dt, int flip, int set_time) {
+  hipLaunchKernelGGL(k_set_ctrl, dim3(1), dim3(64), 0, c->stream, tp<Ctrl>(c->ctrl), t_local, dt,
+                     flip, set_time);
+  HIPCHK(c, hipGetLastError());
+  return MFSV_OK;
+}
+
+bool valid_field(int f) { return f >= 0 && f < MFSV_FIELD_COUNT; }
+
+}  // namespace
+
+extern "C" {
+
+const char* mfsv_version(void) { return "mfsv 0.1 (gfx950)"; }
+
+int mfsv_device_count(int* count) {
+  if (!count) return MFSV_ERR_ARG;
+  if (hipGetDeviceCount(count) != hipSuccess) {
+    *count = 0;
+    return MFSV_ERR_HIP;
+  }
+  return MFSV_OK;
+}
+
+int mfsv_create(const mfsv_grid* g, int precision, int device, mfsv_ctx** out) {
+  if (!g || !out || g->nx < 1 || g->ny < 1 || !(g->dx > 0) || !(g->dy > 0)) return MFSV_ERR_ARG;
+  if (precision != MFSV_F64 && precision != MFSV_F32) return MFSV_ERR_ARG;
+  *out = nullptr;
+  mfsv_ctx* c = new (std::nothrow) mfsv_ctx();
+  if (!c) return MFSV_ERR_NOMEM;
+  c->dev = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return MFSV_ERR_HIP;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MFSV_ERR_HIP;
+  }
+  c->prec = precision;
+  c->vsz = precision == MFSV_F64 ? 8 : 4;
+  c->nx = g->nx; c->ny = g->ny; c->nc = g->nx * g->ny;
+  c->dx = g->dx; c->dy = g->dy; c->ox = g->origin_x; c->oy = g->origin_y;
+  c->P.cn_target = 1.0;
+  c->P.n_shallow = 0.03; c->P.n_deep = 0.3; c->P.d_shallow = 0.1; c->P.d_deep = 0.3;
+  // heun2 by default (RungeKutta.cpp:69-72)
+  c->nstages = 2;
+  c->tab[1][0] = 1.0; c->tab[1][1] = 1.0;
+  c->tab[2][1] = 0.5; c->tab[2][2] = 0.5;
+  geometry(c);
+  int rc = 0;
+  const size_t fb = c->vsz * c->nc;
+  for (int k = 0; k < 2 && !rc; ++k)
+    for (int q = 0; q < 3 && !rc; ++q) rc = alloc(c, c->U[k][q], fb);
+  for (int i = 1; i < c->nstages && !rc; ++i)
+    for (int q = 0; q < 3 && !rc; ++q) rc = alloc(c, c->acc[i][q], fb);
+  if (!rc) rc = alloc(c, c->zb, fb);
+  if (!rc) rc = alloc(c, c->ctrl, sizeof(Ctrl));
+  if (!rc) rc = ensure_partials(c);
+  if (!rc && hipHostMalloc((void**)&c->hctrl, sizeof(Ctrl)) != hipSuccess) rc = MFSV_ERR_HIP;
+  if (!rc && (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess))
+    rc = MFSV_ERR_HIP;
+  if (!rc) {
+    for (int k = 0; k < 2; ++k)
+      for (int q = 0; q < 3; ++q)
+        if (hipMemsetAsync(c->U[k][q].p, 0, fb, c->stream) != hipSuccess) rc = MFSV_ERR_HIP;
+    if (hipMemsetAsync(c->zb.p, 0, fb, c->stream) != hipSuccess) rc = MFSV_ERR_HIP;
+    Ctrl h{};
+    h.step_dur = 60.0; h.dt = 1.0; h.cn_target = 1.0; h.nsteps = 0;
+    h.hb0 = NAN; h.hb1 = NAN;
+    *c->hctrl = h;
+    if (hipMemcpyAsync(c->ctrl.p, c->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      rc = MFSV_ERR_HIP;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) rc = MFSV_ERR_HIP;
+  }
+  if (rc) {
+    mfsv_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return MFSV_OK;
+}
+
+int mfsv_destroy(mfsv_ctx* c) {
+  if (!c) return MFSV_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  for (auto& a : c->U) for (auto& b : a) release(b);
+  for (auto& a : c->acc) for (auto& b : a) release(b);
+  for (Buf* b : {&c->zb, &c->nfield, &c->q0f, &c->q1f, &c->hb0f, &c->hb1f, &c->kf[0], &c->kf[1],
+                 &c->kf[2], &c->brx, &c->bry, &c->ctrl, &c->part_vol, &c->part_rate, &c->forcing,
+                 &c->volp})
+    release(*b);
+  if (c->hctrl) (void)hipHostFree(c->hctrl);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MFSV_OK;
+}
+
+const char* mfsv_last_error(const mfsv_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mfsv_synchronize(mfsv_ctx* c) {
+  if (!c) return MFSV_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int mfsv_device_bytes(const mfsv_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return MFSV_ERR_ARG;
+  *bytes = c->bytes;
+  return MFSV_OK;
+}
+
+int mfsv_set_params(mfsv_ctx* c, const mfsv_params* p) {
+  if (!c || !p) return MFSV_ERR_ARG;
+  if (p->discharge < 0 || p->discharge > 2 || p->head < 0 || p->head > 2)
+    return fail(c, MFSV_ERR_ARG, "discharge/head mode out of range");
+  if (!(p->cn_target > 0)) return fail(c, MFSV_ERR_ARG, "cn_target must be positive");
+  if ((p->flags & MFSV_E4_STORM) && !(p->storm_sigma > 0))
+    return fail(c, MFSV_ERR_ARG, "storm sigma must be positive");
+  c->P = *p;
+  drop_graph(c);
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  // cn_target lives in the control block
+  if (read_ctrl(c)) return MFSV_ERR_HIP;
+  c->hctrl->cn_target = p->cn_target;
+  HIPCHK(c, hipMemcpyAsync(c->ctrl.p, c->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int mfsv_set_scheme(mfsv_ctx* c, int nstages, const double* tab) {
+  if (!c || !tab || nstages < 1 || nstages > kMaxStages) return MFSV_ERR_ARG;
+  for (int i = 0; i <= kMaxStages; ++i)
+    for (int j = 0; j <= kMaxStages; ++j)
+      c->tab[i][j] = (i <= nstages && j <= nstages) ? tab[i * (nstages + 1) + j] : 0.0;
+  c->nstages = nstages;
+  drop_graph(c);
+  int rc = 0;
+  const size_t fb = c->vsz * c->nc;
+  for (int i = 1; i < nstages && !rc; ++i)
+    for (int q = 0; q < 3 && !rc; ++q) rc = alloc(c, c->acc[i][q], fb);
+  for (int i = nstages; i < kMaxStages; ++i)
+    for (int q = 0; q < 3; ++q) {
+      if (c->acc[i][q].p) { c->bytes -= (int64_t)c->acc[i][q].bytes; release(c->acc[i][q]); }
+    }
+  return rc;
+}
+
+int mfsv_set_time(mfsv_ctx* c, double start, double end, double step_duration, double initial_dt) {
+  if (!c || !(step_duration > 0) || !(initial_dt > 0) || !(end >= start)) return MFSV_ERR_ARG;
+  c->start = start; c->end = end; c->step_dur = step_duration; c->dt0 = initial_dt;
+  return MFSV_OK;
+}
+
+int mfsv_start_new_step(mfsv_ctx* c, double time_now, double q0, double q1, double hb0, double hb1) {
+  if (!c) return MFSV_ERR_ARG;
+  if (read_ctrl(c)) return MFSV_ERR_HIP;
+  Ctrl& h = *c->hctrl;
+  h.q0 = q0; h.q1 = q1; h.hb0 = hb0; h.hb1 = hb1;
+  h.gstep = time_now;
+  h.step_dur = c->step_dur;
+  h.start_time = c->start;
+  h.done = 0;
+  HIPCHK(c, hipMemcpyAsync(c->ctrl.p, c->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int mfsv_set_forcing_table(mfsv_ctx* c, int64_t nsteps, const double* table) {
+  if (!c || nsteps < 0 || (nsteps && !table)) return MFSV_ERR_ARG;
+  drop_graph(c);
+  if (nsteps == 0) {
+    c->nforcing = 0;
+    return MFSV_OK;
+  }
+  int rc = alloc(c, c->forcing, sizeof(double) * 4 * nsteps);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->forcing.p, table, sizeof(double) * 4 * nsteps, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->nforcing = nsteps;
+  return MFSV_OK;
+}
+
+int mfsv_upload(mfsv_ctx* c, int field, const double* host, int64_t n) {
+  if (!c || !host || !valid_field(field)) return MFSV_ERR_ARG;
+  if (n != c->nc) return fail(c, MFSV_ERR_ARG, "count must equal nx*ny");
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  switch (field) {
+    case MFSV_FIELD_H: case MFSV_FIELD_U: case MFSV_FIELD_V:
+      return put(c, c->U[c->cur][field], host, n, 0.0);
+    case MFSV_FIELD_ZB: {
+      double off = 0.0;
+      if (c->prec == MFSV_F32) {  // depth-based form: bed relative to its minimum
+        double m = INFINITY;
+        for (int64_t i = 0; i < n; ++i)
+          if (host[i] < m) m = host[i];
+        off = std::isfinite(m) ? m : 0.0;
+      }
+      c->zoff = off;
+      return put(c, c->zb, host, n, off);
+    }
+    case MFSV_FIELD_MANNING_N: {
+      int rc = put(c, c->nfield, host, n, 0.0);
+      if (!rc) c->nfield_set = true;
+      return rc;
+    }
+    case MFSV_FIELD_QBDY0: return put(c, c->q0f, host, n, 0.0);
+    case MFSV_FIELD_QBDY1: return put(c, c->q1f, host, n, 0.0);
+    case MFSV_FIELD_HBDY0: return put(c, c->hb0f, host, n, 0.0);
+    case MFSV_FIELD_HBDY1: return put(c, c->hb1f, host, n, 0.0);
+    default: return fail(c, MFSV_ERR_ARG, "field is not uploadable");
+  }
+}
+
+int mfsv_download(mfsv_ctx* c, int field, double* host, int64_t n) {
+  if (!c || !host || !valid_field(field)) return MFSV_ERR_ARG;
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  const int64_t nbx = (c->nx + 1) * c->ny, nby = c->nx * (c->ny + 1);
+  if (field == MFSV_FIELD_BRANCH_X || field == MFSV_FIELD_BRANCH_Y) {
+    const int64_t want = field == MFSV_FIELD_BRANCH_X ? nbx : nby;
+    if (n != want) return fail(c, MFSV_ERR_ARG, "branch field count mismatch");
+    const Buf& b = field == MFSV_FIELD_BRANCH_X ? c->brx : c->bry;
+    return get(c, b, host, n, 0.0, 4);
+  }
+  if (n != c->nc) return fail(c, MFSV_ERR_ARG, "count must equal nx*ny");
+  switch (field) {
+    case MFSV_FIELD_H: case MFSV_FIELD_U: case MFSV_FIELD_V:
+      return get(c, c->U[c->cur][field], host, n, 0.0, c->vsz);
+    case MFSV_FIELD_ZB: return get(c, c->zb, host, n, c->zoff, c->vsz);
+    case MFSV_FIELD_MANNING_N: return get(c, c->nfield, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_QBDY0: return get(c, c->q0f, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_QBDY1: return get(c, c->q1f, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_HBDY0: return get(c, c->hb0f, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_HBDY1: return get(c, c->hb1f, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_DHDT: case MFSV_FIELD_DUDT: case MFSV_FIELD_DVDT:
+      return get(c, c->kf[field - MFSV_FIELD_DHDT], host, n, 0.0, c->vsz);
+    default: return MFSV_ERR_ARG;
+  }
+}
+
+static int check_inputs(mfsv_ctx* c) {
+  if (c->P.discharge == 2 && (!c->q0f.p || !c->q1f.p))
+    return fail(c, MFSV_ERR_STATE, "per-cell discharge needs QBDY0 and QBDY1");
+  if (c->P.head == 2 && (!c->hb0f.p || !c->hb1f.p))
+    return fail(c, MFSV_ERR_STATE, "per-cell head needs HBDY0 and HBDY1");
+  return MFSV_OK;
+}
+
+int mfsv_dudt(mfsv_ctx* c, double t_sub, double dt, int want_branches) {
+  if (!c || !(dt > 0)) return MFSV_ERR_ARG;
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  int rc;
+  if ((rc = check_inputs(c))) return rc;
+  for (int q = 0; q < 3; ++q)
+    if ((rc = alloc(c, c->kf[q], c->vsz * c->nc))) return rc;
+  if (want_branches) {
+    if ((rc = alloc(c, c->brx, 4 * (c->nx + 1) * c->ny))) return rc;
+    if ((rc = alloc(c, c->bry, 4 * c->nx * (c->ny + 1)))) return rc;
+  }
+  // stage 0 has c = 0, so t_local = t_sub gives the stage time t_sub
+  if ((rc = set_ctrl(c, t_sub, dt, 0, 1))) return rc;
+  if ((rc = launch_stage(c, 0, true, want_branches != 0))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int mfsv_timestep(mfsv_ctx* c, double t_local, double dt, double* cn) {
+  if (!c || !(dt > 0) || !cn) return MFSV_ERR_ARG;
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  int rc;
+  if ((rc = check_inputs(c))) return rc;
+  if ((rc = set_ctrl(c, t_local, dt, 0, 1))) return rc;
+  if ((rc = launch_timestep(c, 1))) return rc;
+  if ((rc = read_ctrl(c))) return rc;
+  *cn = c->hctrl->cn;
+  return MFSV_OK;
+}
+
+int mfsv_accept(mfsv_ctx* c) {
+  if (!c) return MFSV_ERR_ARG;
+  int rc = set_ctrl(c, 0, 0, 1, 0);
+  if (rc) return rc;
+  c->cur ^= 1;
+  return MFSV_OK;
+}
+
+int mfsv_volume(mfsv_ctx* c, double* v) {
+  if (!c || !v) return MFSV_ERR_ARG;
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  return compute_volume(c, c->cur, v);
+}
+
+// TimeParameters.cpp:40-58
+static int64_t num_steps(const mfsv_ctx* c) {
+  const double eps = 1e-6;
+  const double dur = c->end - c->start;
+  const double frac = dur / c->step_dur;
+  const int64_t n = (int64_t)frac;
+  if (c->step_dur * n >= dur - eps && c->step_dur * n <= dur + eps) return n;
+  return n + 1;
+}
+
+int mfsv_run_begin(mfsv_ctx* c) {
+  if (!c) return MFSV_ERR_ARG;
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  int rc;
+  if ((rc = check_inputs(c))) return rc;
+  const int64_t ns = num_steps(c);
+  if ((c->P.discharge == 1 || c->P.head == 1) && c->nforcing < ns)
+    return fail(c, MFSV_ERR_STATE, "scalar discharge/head needs a forcing table row per outer step");
+  if ((rc = compute_volume(c, c->cur, &c->volume0))) return rc;
+  if ((rc = read_ctrl(c))) return rc;
+  Ctrl& h = *c->hctrl;
+  h.t_local = 0.0;
+  h.dt = c->dt0;
+  h.step_dur = c->step_dur;
+  h.start_time = c->start;
+  h.gstep = c->start;
+  h.cn = 0.0;
+  h.cn_target = c->P.cn_target;
+  h.volume = c->volume0;
+  h.srcvol = 0.0;
+  h.step = 0;
+  h.nsteps = ns;
+  h.n_acc = 0; h.n_rej = 0;
+  h.cn_bits = 0;
+  h.cur = c->cur;
+  h.done = ns <= 0 ? 1 : 0;
+  h.repeat = 0;
+  if (c->nforcing) {
+    std::vector<double> f0(4);
+    HIPCHK(c, hipMemcpy(f0.data(), c->forcing.p, 32, hipMemcpyDeviceToHost));
+    h.q0 = f0[0]; h.q1 = f0[1]; h.hb0 = f0[2]; h.hb1 = f0[3];
+  } else {
+    h.q0 = 0; h.q1 = 0; h.hb0 = NAN; h.hb1 = NAN;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->ctrl.p, c->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+static int build_graph(mfsv_ctx* c, int64_t steps) {
+  drop_graph(c);
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int64_t k = 0; k < steps && !rc; ++k) rc = launch_timestep(c, 0);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(c, MFSV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  c->graph = g;
+  HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  c->graph_steps = steps;
+  return MFSV_OK;
+}
+
+int mfsv_run(mfsv_ctx* c, int64_t max_timesteps, int use_graph) {
+  if (!c || max_timesteps < 0) return MFSV_ERR_ARG;
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  int rc;
+  const int64_t batch = 32;
+  if (use_graph && c->graph_steps != batch)
+    if ((rc = build_graph(c, batch))) return rc;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  int64_t left = max_timesteps;
+  while (left > 0) {
+    const int64_t n = left < batch ? left : batch;
+    if (use_graph && n == batch) {
+      HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    } else {
+      for (int64_t k = 0; k < n; ++k)
+        if ((rc = launch_timestep(c, 0))) return rc;
+    }
+    left -= n;
+    if ((rc = read_ctrl(c))) return rc;
+    if (c->hctrl->done) break;
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  if ((rc = read_ctrl(c))) return rc;
+  c->cur = c->hctrl->cur;
+  return MFSV_OK;
+}
+
+int mfsv_get_stats(mfsv_ctx* c, mfsv_stats* s) {
+  if (!c || !s) return MFSV_ERR_ARG;
+  int rc = read_ctrl(c);
+  if (rc) return rc;
+  const Ctrl& h = *c->hctrl;
+  s->time = h.gstep + h.t_local;
+  s->t_local = h.t_local;
+  s->dt = h.dt;
+  s->cn = h.cn;
+  s->volume = h.volume;
+  s->source_volume = h.srcvol;
+  s->volume0 = c->volume0;
+  s->step = h.step;
+  s->nsteps = h.nsteps;
+  s->accepted = h.n_acc;
+  s->rejected = h.n_rej;
+  s->done = h.done;
+  s->current = h.cur;
+  c->cur = h.cur;
+  return MFSV_OK;
+}
+
+int mfsv_last_run_ms(mfsv_ctx* c, double* ms) {
+  if (!c || !ms) return MFSV_ERR_ARG;
+  *ms = c->last_ms;
+  return MFSV_OK;
+}
+
+}  // extern "C"

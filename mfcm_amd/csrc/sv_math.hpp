@@ -1,0 +1,231 @@
+// sv_math.hpp -- device arithmetic of the Saint-Venant stage (gfx950).
+//
+// Restates the per-point arithmetic of mfcm's Cartesian-mesh solver:
+//   Minmod3                SpatialDerivative/Minmod3.hpp:34-82
+//   face flux (4 branches) SaintVenant/FluxKernel.cpp:66-295
+// For T = double the operation order is the reference's, term by term, so
+// the FP64 path reproduces it to the last bit (the file is compiled with
+// -ffp-contract=off; the only fused operations are the explicit fma()s of
+// div_c, which return the correctly rounded quotient).  For T = float the
+// same routines form the precision form, whose wet/dry tests compare depths
+// with bed steps instead of summed surface elevations.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mfsv {
+
+constexpr double kG = 9.81;
+
+template <typename T> struct Num;
+template <> struct Num<double> {
+  static __device__ __forceinline__ double nan() { return __builtin_nan(""); }
+};
+template <> struct Num<float> {
+  static __device__ __forceinline__ float nan() { return __builtin_nanf(""); }
+};
+
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double sqrt_t(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float sqrt_t(float a) { return __builtin_sqrtf(a); }
+__device__ __forceinline__ double fabs_t(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float fabs_t(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double fmax_t(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float fmax_t(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double fmin_t(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float fmin_t(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ bool isnan_t(double a) { return __builtin_isnan(a); }
+__device__ __forceinline__ bool isnan_t(float a) { return __builtin_isnan(a); }
+
+// a / b for a divisor b known on the host with rb = 1/b rounded to nearest:
+// q = a*rb, then one residual step with exact fma.  For binary64 this is the
+// correctly rounded quotient (Markstein's correction with a correctly
+// rounded reciprocal; brute-force checked on 3.5e8 operand pairs), so it
+// equals the reference's a / b wherever neither overflows nor underflows.
+template <typename T>
+__device__ __forceinline__ T div_c(T a, T b, T rb) {
+  T q = a * rb;
+  T e = fma_t(-q, b, a);
+  return fma_t(e, rb, q);
+}
+
+// x^(4/3) for the Manning term (sycl::pow(inv_h, 4/3),
+// ManningRoughnessSourceTerm.cpp:66-67) from IEEE operations only, so the
+// CPU oracle computes the same bits: five Newton steps for r = x^(-1/3) from
+// an exponent-bit estimate, then (x r)^2.  Within 7 ulp of x^(4/3) for the
+// Manning range x in (1e-3, 16).
+__device__ __forceinline__ double pow43(double x) {
+  unsigned long long i = (unsigned long long)__double_as_longlong(x);
+  i = 0x5540000000000000ULL - i / 3ull;
+  double r = __longlong_as_double((long long)i);
+  const double third = 1.0 / 3.0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const double r3 = r * r * r;
+    const double t = 4.0 - x * r3;
+    r = r * t * third;
+  }
+  const double c = x * r;
+  return c * c;
+}
+// FP32 form: hardware log2/exp2.
+__device__ __forceinline__ float pow43(float x) {
+  return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (4.0f / 3.0f));
+}
+
+// sycl::sign: 1 for x > 0, -1 for x < 0, the signed zero itself, 0 for NaN.
+template <typename T>
+__device__ __forceinline__ T sy_sign(T x) {
+  return x > T(0) ? T(1) : (x < T(0) ? T(-1) : (x != x ? T(0) : x));
+}
+// sycl::min on floating point: y < x ? y : x.
+template <typename T>
+__device__ __forceinline__ T sy_min(T x, T y) { return (y < x) ? y : x; }
+
+// Minmod3 with theta = 2 on a uniform axis.  A missing neighbour is the
+// cell itself at distance 0 (Cartesian2DMesh.hpp:477-520): pass its value as
+// Uc with has = false.  d/rd: the axis spacing and its reciprocal.
+template <typename T>
+__device__ __forceinline__ T minmod3(T Ul, bool hasl, T Uc, bool hasr, T Ur, T d, T rd) {
+  T lhs = hasl ? div_c(T(2) * (Uc - Ul), d, rd) : Num<T>::nan();
+  T rhs = hasr ? div_c(T(2) * (Ur - Uc), d, rd) : Num<T>::nan();
+  T central = T(0);
+  if (hasl || hasr) {
+    T q = div_c(Ur - Ul, d, rd);  // (Ur-Ul)/(2d) == 0.5*((Ur-Ul)/d) exactly
+    central = (hasl && hasr) ? T(0.5) * q : q;
+  }
+  T outer;
+  if (isnan_t(lhs)) outer = rhs;
+  else if (isnan_t(rhs)) outer = lhs;
+  else outer = T(0.5) * (sy_sign(lhs) + sy_sign(rhs)) * sy_min(fabs_t(lhs), fabs_t(rhs));
+  if (isnan_t(outer)) return central;
+  return T(0.5) * (sy_sign(outer) + sy_sign(central)) * sy_min(fabs_t(outer), fabs_t(central));
+}
+
+// One side of a face: cell values and the limited slopes along the face
+// normal.  un/ut: velocity normal/tangential to the face (u/v on x-faces,
+// v/u on y-faces); s*: the matching slopes.
+template <typename T> struct Side {
+  T h, un, ut, z;
+  T sh, sun, sut, sz;
+};
+
+// Face flux in normal/tangential form: h, normal momentum, tangential
+// momentum, wall force ("z-flux").
+template <typename T> struct Flux {
+  T h, n, t, z;
+};
+
+// FluxKernel.cpp:66-295 for one face.  The tangential-slope terms of the
+// reference's projections are multiplied by a zero direction flag and only
+// add a signed zero, so they are left out.  hd = 0.5*d, rd = 1/d.
+// DEPTHREL: wet tests as h_m > max(zb_p - zb_m, 0) (equal to
+// zb_m + h_m > max(zb_m, zb_p) in exact arithmetic) for the FP32 form.
+template <typename T, bool DEPTHREL>
+__device__ __forceinline__ Flux<T> face_flux(Side<T> L, Side<T> R, int edge, T hd, T d, T rd,
+                                             float* branch) {
+  Flux<T> F;
+  if (L.z != L.z) {                          // FluxKernel.cpp:85-95
+    L = R;
+    if (edge == 1) goto zero;
+    edge = -1;
+  }
+  if (R.z != R.z) {                          // :96-107
+    if (edge == -1) goto zero;
+    R = L;
+    edge = 1;
+  }
+  {
+    const T mL = edge < 0 ? T(0) : T(1);     // :111-148
+    const T mR = edge > 0 ? T(0) : T(1);
+    T hL = L.h * mL, hR = R.h * mR;
+    T unL = L.un * mL, unR = R.un * mR;
+    T shL = L.sh * mL, shR = R.sh * mR;
+    T sunL = L.sun * mL, sunR = R.sun * mR;
+    T szL = L.sz * mL, szR = R.sz * mR;
+    T zL = L.z, zR = R.z;
+    if (edge < 0) zL = zR + hR * T(2);       // :152-157
+    if (edge > 0) zR = zL + hL * T(2);
+    // :161-187
+    const T zm = zL + hd * szL;
+    T hm = hL + hd * shL;
+    const T unm = unL + hd * sunL;
+    const T utm = L.ut + hd * L.sut;
+    const T zp = zR - hd * szR;
+    T hp = hR - hd * shR;
+    const T unp = unR - hd * sunR;
+    const T utp = R.ut - hd * R.sut;
+    bool wm, wp;
+    if (DEPTHREL) {
+      const T dz = zp - zm;
+      wm = hm > fmax_t(dz, T(0));
+      wp = hp > fmax_t(-dz, T(0));
+    } else {
+      const T zf = fmax_t(zm, zp);           // :191-195
+      const T ym = zm + hm;
+      const T yp = zp + hp;
+      wm = ym > zf;
+      wp = yp > zf;
+    }
+    hm = fmax_t(hm, T(0));                   // :199-204
+    hp = fmax_t(hp, T(0));
+    const T G = T(kG);
+    if (wm && wp) {                          // :208-234
+      const T cm = sqrt_t(G * hm);
+      const T cp = sqrt_t(G * hp);
+      const T hfm = hm * unm, hfp = hp * unp;
+      const T nfm = unm * (T(0.5) * unm) + G * hm;
+      const T nfp = unp * (T(0.5) * unp) - G * hp;
+      const T tfm = utm * unm, tfp = utp * unp;
+      const T a = fmax_t(fabs_t(unp + sy_sign(unp) * cp), fabs_t(unm + sy_sign(unm) * cm));
+      const T ha = T(0.5) * a;
+      F.h = T(0.5) * (hfp + hfm) - ha * (hp - hm);
+      F.n = T(0.5) * (nfp + nfm) - ha * (unp - unm);
+      F.t = T(0.5) * (tfp + tfm) - ha * (utp - utm);
+      F.z = (zm - zp) * G;
+      *branch = 1.0f;
+    } else if (!wm && !wp) {                 // :235-254
+      F.h = T(0);
+      F.t = T(0);
+      if (zp > zm) {
+        F.n = T(0.5) * (G * hm);
+        F.z = -hm * T(0.5) * G;
+        *branch = 2.25f;
+      } else {
+        F.n = T(-0.5) * (G * hp);
+        F.z = hp * T(0.5) * G;
+        *branch = 2.75f;
+      }
+    } else if (wm) {                         // :255-271
+      const T cm = sqrt_t(G * hm);
+      const T hfm = hm * unm;
+      const T nfm = unm * (T(0.5) * unm) + G * hm;
+      const T tfm = utm * unm;
+      const T ha = T(0.5) * fabs_t(unm + sy_sign(unm) * cm);
+      F.h = T(0.5) * hfm - ha * (-hm);
+      F.n = T(0.5) * nfm - ha * (-unm);
+      F.t = T(0.5) * tfm - ha * (-utm);
+      F.z = div_c(hp, d, rd);
+      *branch = 3.0f;
+    } else {                                 // :272-289
+      const T cp = sqrt_t(G * hp);
+      const T hfp = hp * unp;
+      const T nfp = unp * (T(0.5) * unp) + G * hp;
+      const T tfp = utp * unp;
+      const T ha = T(0.5) * fabs_t(unp + sy_sign(unp) * cp);
+      F.h = T(0.5) * hfp - ha * hp;
+      F.n = T(0.5) * nfp - ha * unp;
+      F.t = T(0.5) * tfp - ha * utp;
+      F.z = div_c(hm, d, rd);
+      *branch = 4.0f;
+    }
+    return F;
+  }
+zero:                                        // :47-60
+  F.h = T(0); F.n = T(0); F.t = T(0); F.z = T(0);
+  *branch = 0.0f;
+  return F;
+}
+
+}  // namespace mfsv

@@ -1,0 +1,329 @@
+"""ctypes front end of the CPU oracle (oracle/libsvoracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py may import this module; it is the checker for
+the HIP path and never the thing measured or shipped.
+
+The oracle restates mfcm's Cartesian-mesh Saint-Venant solver in FP64 (see
+oracle/sv_oracle.h for the function list, the reference file:line each one
+follows, and what pins it).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsvoracle.so")
+
+E1_RAIN_GUARD = 0x01
+E2_WEST_OUTFLOW = 0x02
+E4_STORM = 0x08
+E7_Y_ORIENT = 0x40
+
+# Named Butcher tables (TemporalScheme/RungeKutta.cpp:54-116), rows as the
+# reference's a_ vectors.
+TABLEAUX = {
+    "euler": [[0.0], [0.0, 1.0]],
+    "midpoint": [[0.0], [0.5, 0.5], [0.0, 0.0, 1.0]],
+    "heun2": [[0.0], [1.0, 1.0], [0.0, 0.5, 0.5]],
+    "ralston2": [[0.0], [2.0 / 3.0, 2.0 / 3.0], [0.0, 0.25, 0.75]],
+    "kutta3": [[0.0], [0.5, 0.5], [1.0, -1.0, 2.0], [0.0, 1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0]],
+    "heun3": [[0.0], [1.0 / 3.0, 1.0 / 3.0], [2.0 / 3.0, 0.0, 2.0 / 3.0], [0.0, 0.25, 0.0, 0.75]],
+    "wray3": [[0.0], [8.0 / 15.0, 8.0 / 15.0], [2.0 / 3.0, 0.25, 5.0 / 12.0], [0.0, 0.25, 0.0, 0.75]],
+    "ralston3": [[0.0], [0.5, 0.5], [0.75, 0.0, 0.75], [0.0, 2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0]],
+    "ssprk3": [[0.0], [1.0, 1.0], [0.5, 0.25, 0.25], [0.0, 1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0]],
+    "classic": [[0.0], [0.5, 0.5], [0.5, 0.0, 0.5], [1.0, 0.0, 0.0, 1.0],
+                [0.0, 1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0]],
+    "ralston4": [[0.0], [0.4, 0.4], [0.45573725, 0.29697761, 0.15875964],
+                 [1.0, 0.21810040, -3.05096516, 3.83286476],
+                 [0.0, 0.17476028, -0.55148066, 1.20553560, 0.17118478]],
+    "3/8rule": [[0.0], [1.0 / 3.0, 1.0 / 3.0], [2.0 / 3.0, -1.0 / 3.0, 1.0],
+                [1.0, 1.0, -1.0, 1.0], [0.0, 1.0 / 8.0, 3.0 / 8.0, 3.0 / 8.0, 1.0 / 8.0]],
+}
+
+
+def tableau_matrix(name_or_rows):
+    rows = TABLEAUX[name_or_rows] if isinstance(name_or_rows, str) else name_or_rows
+    n = len(rows) - 1
+    m = np.zeros((n + 1, n + 1), dtype=np.float64)
+    for i, r in enumerate(rows):
+        m[i, : len(r)] = r
+    return n, m
+
+
+class CellV(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "h", "u", "v", "zb", "dzbdx", "dzbdy", "dhdx", "dudx", "dvdx",
+        "dhdy", "dudy", "dvdy")]
+
+
+class Storm(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("peak", "sigma", "cx0", "cy0", "vx", "vy")]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        d, i64, p = C.c_double, C.c_int64, C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.or_minmod3.restype = d
+        L.or_minmod3.argtypes = [d, d, d, d, d]
+        L.or_face.restype = None
+        L.or_face.argtypes = [C.POINTER(CellV), C.POINTER(CellV), C.c_int, C.c_int, d, dp]
+        L.or_slope_x.argtypes = [i64, i64, d, p, p]
+        L.or_slope_y.argtypes = [i64, i64, d, p, p, C.c_int]
+        L.or_storm_rate.restype = d
+        L.or_storm_rate.argtypes = [C.POINTER(Storm), d, d, d]
+        L.or_sim_create.restype = p
+        L.or_sim_create.argtypes = [i64, i64, d, d, C.c_int, p, C.c_int, d]
+        L.or_sim_destroy.argtypes = [p]
+        L.or_sim_set_bed.argtypes = [p, p]
+        L.or_sim_set_state.argtypes = [p, p, p, p]
+        L.or_sim_get_state.argtypes = [p, C.c_int, p, p, p]
+        L.or_sim_set_manning.argtypes = [p, p, p, p, p]
+        L.or_sim_add_discharge.argtypes = [p, C.c_int, p, p, p]
+        L.or_sim_add_head.argtypes = [p, C.c_int, p, p, p]
+        L.or_sim_set_storm.argtypes = [p, C.POINTER(Storm), d, d]
+        L.or_sim_set_time.argtypes = [p, d, d, d, d]
+        L.or_sim_num_steps.restype = i64
+        L.or_sim_num_steps.argtypes = [p]
+        L.or_sim_start_new_step.argtypes = [p, d]
+        L.or_sim_update_dudt.argtypes = [p, C.c_int, d, d]
+        L.or_sim_update_y.argtypes = [p, C.c_int, d]
+        L.or_sim_control_number.restype = d
+        L.or_sim_control_number.argtypes = [p, C.c_int, d]
+        L.or_sim_do_timestep.restype = C.c_int
+        L.or_sim_do_timestep.argtypes = [p, d, d, dp]
+        L.or_sim_accept.argtypes = [p]
+        L.or_sim_step.argtypes = [p]
+        L.or_sim_solve.argtypes = [p]
+        L.or_sim_run_steps.argtypes = [p, i64, i64]
+        L.or_sim_dt.restype = d
+        L.or_sim_dt.argtypes = [p]
+        L.or_sim_set_dt.argtypes = [p, d]
+        L.or_sim_stats.argtypes = [p, p]
+        L.or_sim_volume.restype = d
+        L.or_sim_volume.argtypes = [p, C.c_int]
+        L.or_sim_set_trace.argtypes = [p, p, i64]
+        L.or_sim_trace_len.restype = i64
+        L.or_sim_trace_len.argtypes = [p]
+        L.or_sim_array.restype = dp
+        L.or_sim_array.argtypes = [p, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def minmod3(ul, dxl, uc, dxr, ur):
+    return lib().or_minmod3(ul, dxl, uc, dxr, ur)
+
+
+def face(L: dict, R: dict, edge: int, dir: int, dx: float):
+    out = (C.c_double * 5)()
+    cl, cr = CellV(**L), CellV(**R)
+    lib().or_face(C.byref(cl), C.byref(cr), edge, dir, dx, out)
+    return list(out)
+
+
+def slope_x(f, dx):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty_like(f)
+    lib().or_slope_x(f.shape[1], f.shape[0], dx, _ptr(f), _ptr(out))
+    return out
+
+
+def slope_y(f, dy, flags=0):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty_like(f)
+    lib().or_slope_y(f.shape[1], f.shape[0], dy, _ptr(f), _ptr(out), flags)
+    return out
+
+
+def storm_rate(storm: dict, x, y, t):
+    s = Storm(**storm)
+    return lib().or_storm_rate(C.byref(s), x, y, t)
+
+
+# Indices for Sim.array()
+ARR = {"h": 0, "u": 1, "v": 2, "dhdt": 3, "dudt": 4, "dvdt": 5,
+       "dhdx": 6, "dudx": 7, "dvdx": 8, "dhdy": 9, "dudy": 10, "dvdy": 11,
+       "zb": 12, "dzbdx": 13, "dzbdy": 14, "hflux": 15, "uflux": 16,
+       "vflux": 17, "zflux": 18, "branch": 19, "qbdy0": 20, "qbdy1": 21,
+       "hbdy0": 22, "hbdy1": 23, "mannings_n": 24, "friction_slope": 25}
+
+
+class Sim:
+    """FP64 restatement of RungeKuttaSolver<SaintVenantSolver> on one mesh."""
+
+    def __init__(self, nx, ny, dx, dy, scheme="heun2", flags=0, cn_target=1.0):
+        self.nx, self.ny, self.dx, self.dy = int(nx), int(ny), float(dx), float(dy)
+        self.nstages, tab = tableau_matrix(scheme)
+        self._tab = np.ascontiguousarray(tab)
+        self.flags = flags
+        self._h = lib().or_sim_create(self.nx, self.ny, self.dx, self.dy, self.nstages,
+                                      _ptr(self._tab), flags, cn_target)
+        if not self._h:
+            raise ValueError("or_sim_create failed")
+        self._keep = []
+        self.nc = self.nx * self.ny
+        self.nf = (self.nx + 1) * self.ny + (self.ny + 1) * self.nx
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().or_sim_destroy(h)
+            self._h = None
+
+    def set_bed(self, zb):
+        lib().or_sim_set_bed(self._h, _ptr(_f64(zb, self.nc)))
+
+    def set_state(self, h, u=None, v=None):
+        z = np.zeros(self.nc)
+        h = _f64(h, self.nc)
+        u = z if u is None else _f64(u, self.nc)
+        v = z if v is None else _f64(v, self.nc)
+        lib().or_sim_set_state(self._h, _ptr(h), _ptr(u), _ptr(v))
+
+    def get_state(self, idx=0):
+        h, u, v = (np.empty(self.nc) for _ in range(3))
+        lib().or_sim_get_state(self._h, idx, _ptr(h), _ptr(u), _ptr(v))
+        shp = (self.ny, self.nx)
+        return h.reshape(shp), u.reshape(shp), v.reshape(shp)
+
+    def set_manning(self, n_sh, n_dp=None, d_sh=0.1, d_dp=0.3):
+        n_dp = n_sh if n_dp is None else n_dp
+        arrs = [np.broadcast_to(np.asarray(a, np.float64), (self.ny, self.nx)).copy()
+                for a in (n_sh, n_dp, d_sh, d_dp)]
+        lib().or_sim_set_manning(self._h, *[_ptr(a) for a in arrs])
+
+    def _series(self, fn, t, v, sel):
+        t = _f64(t)
+        v = _f64(v)
+        s = None
+        if sel is not None:
+            s = np.ascontiguousarray(np.asarray(sel, dtype=np.uint8).ravel())
+            if s.size != self.nc:
+                raise ValueError("selection size")
+        rc = fn(self._h, t.size, _ptr(t), _ptr(v), None if s is None else _ptr(s))
+        if rc != 0:
+            raise ValueError("too many boundary blocks or short series")
+
+    def add_discharge(self, t, v, sel=None):
+        self._series(lib().or_sim_add_discharge, t, v, sel)
+
+    def add_head(self, t, v, sel=None):
+        self._series(lib().or_sim_add_head, t, v, sel)
+
+    def set_storm(self, storm: dict, origin=(0.0, 0.0)):
+        s = Storm(**storm)
+        lib().or_sim_set_storm(self._h, C.byref(s), origin[0], origin[1])
+
+    def set_time(self, start, end, step_duration=60.0, initial_dt=1.0):
+        lib().or_sim_set_time(self._h, start, end, step_duration, initial_dt)
+
+    def num_steps(self):
+        return lib().or_sim_num_steps(self._h)
+
+    def start_new_step(self, t):
+        lib().or_sim_start_new_step(self._h, t)
+
+    def update_dudt(self, s, t_sub, dt):
+        lib().or_sim_update_dudt(self._h, s, t_sub, dt)
+
+    def update_y(self, step, dt):
+        lib().or_sim_update_y(self._h, step, dt)
+
+    def control_number(self, s, dt):
+        return lib().or_sim_control_number(self._h, s, dt)
+
+    def do_timestep(self, t_local, dt):
+        nd = C.c_double()
+        rep = lib().or_sim_do_timestep(self._h, t_local, dt, C.byref(nd))
+        return nd.value, bool(rep)
+
+    def accept(self):
+        lib().or_sim_accept(self._h)
+
+    def step(self):
+        lib().or_sim_step(self._h)
+
+    def solve(self):
+        lib().or_sim_solve(self._h)
+
+    def run_steps(self, first, count):
+        lib().or_sim_run_steps(self._h, first, count)
+
+    @property
+    def dt(self):
+        return lib().or_sim_dt(self._h)
+
+    @dt.setter
+    def dt(self, v):
+        lib().or_sim_set_dt(self._h, v)
+
+    def stats(self):
+        s = np.zeros(2, dtype=np.int64)
+        lib().or_sim_stats(self._h, _ptr(s))
+        return int(s[0]), int(s[1])
+
+    def volume(self, idx=0):
+        return lib().or_sim_volume(self._h, idx)
+
+    def enable_trace(self, cap=1 << 20):
+        self._trace = np.zeros((cap, 4), dtype=np.float64)
+        lib().or_sim_set_trace(self._h, _ptr(self._trace), cap)
+
+    def trace(self):
+        n = min(lib().or_sim_trace_len(self._h), self._trace.shape[0])
+        return self._trace[:n].copy()
+
+    def array(self, name, idx=0):
+        which = ARR[name]
+        n = self.nf if 15 <= which <= 19 else self.nc
+        p = lib().or_sim_array(self._h, which, idx)
+        a = np.ctypeslib.as_array(p, shape=(n,)).copy()
+        return a
+
+
+def sim_from_scenario(sc):
+    """Oracle solver set up from a mfcm_amd.scenarios.Scenario (the same
+    configuration the HIP path receives)."""
+    sim = Sim(sc.nx, sc.ny, sc.dx, sc.dy, sc.scheme, sc.flags, sc.cn_target)
+    sim.set_bed(sc.zb)
+    sim.set_state(sc.h0, sc.u0, sc.v0)
+    if sc.n_manning is not None:
+        sim.set_manning(sc.n_manning, sc.n_manning, sc.d_shallow, sc.d_deep)
+    if sc.rain_t is not None:
+        sim.add_discharge(sc.rain_t, sc.rain_v)
+    if sc.head_cols:
+        sel = np.zeros((sc.ny, sc.nx), np.uint8)
+        sel[:, : sc.head_cols] = 1
+        sim.add_head([0.0, 1e12], [0.0, 0.0], sel)
+    if sc.storm is not None:
+        sim.set_storm(sc.storm.as_dict())
+    sim.set_time(0.0, sc.t_end, sc.step_duration, sc.initial_dt)
+    return sim

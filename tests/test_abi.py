@@ -1,0 +1,44 @@
+"""The C-ABI library loads and exports every symbol include/mfsv.h declares.
+
+CPU-only: loading libmfsv_hip.so needs no GPU; no compute entry point is
+called here (device queries return an error or zero devices without one).
+"""
+import os
+import re
+
+from mfcm_amd import mfsv
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "mfsv.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mfsv_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_declares_the_abi():
+    syms = declared_symbols()
+    assert "mfsv_create" in syms and "mfsv_run" in syms
+    assert len(syms) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    L = mfsv.lib()
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_the_header():
+    assert set(declared_symbols()) == set(mfsv.SIGNATURES)
+
+
+def test_version_string():
+    assert b"gfx950" in mfsv.lib().mfsv_version()
+
+
+def test_argument_errors_without_device():
+    # null arguments are rejected before any device call
+    assert mfsv.lib().mfsv_create(None, 0, 0, None) == -1
+    assert mfsv.lib().mfsv_destroy(None) == -1
+    assert mfsv.lib().mfsv_device_count(None) == -1
