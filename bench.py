@@ -1,0 +1,259 @@
+"""Throughput benchmark of the explicit 2-D Saint-Venant time step on one MI355X.
+
+Workload (the headline configuration): one 2048 x 2048 catchment, dx = 2 m,
+tilted terrain with 30 Gaussian hills and 1 m fBm on a +300 m base, Manning n
+from 4 Voronoi land-use classes in [0.02, 0.08], a drifting Gaussian storm
+(80 mm/h peak, sigma 400 m, 5 m/s), Heun-2, CFL 0.5, free outflow on the
+x = 0 edge, no velocity in cells shallower than 1 mm, velocity components
+within 10 m/s (extension flags E1 E2 E4 E7 E8 E9), dry start.
+
+A "step" is one accepted-or-rejected timestep of the whole catchment: both
+RK stages (fused stage kernels) plus the device controller.  The run starts
+from the dry initial state; W warmup timesteps are untimed, then exactly K
+timesteps are timed between device synchronisations.  Inputs are resident
+in HBM before timing starts.
+
+value = accepted cell-updates per second of the FP32 precision form (FP64
+time, controller and mass balance); the FP64 plain method is measured the
+same way and reported beside it, with the FP32-vs-FP64 depth, velocity and
+mass errors after a common simulated horizon.
+
+roofline: the dominant kernel is the stage kernel.  Algorithmic bytes per
+cell-update B = s*(15 + 2m) with m = 2 streamed static fields (bed, n):
+152 B in FP64, 76 B in FP32; one stage launch processes the N cells of the
+grid, so achieved = (B/2)*N / (mean stage-launch duration from HIP events on
+the library's stream).
+
+cpu_baseline: the CPU oracle (a C restatement of the reference's kernel
+sequence, FP64, OpenMP) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s & HBM GB/s, FP32-batched vs FP64 plain; p50/p99 scenario latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_scenario(n):
+    from mfcm_amd import scenarios as S
+    sc = S.config1(seed=1, n=n, dx=2.0, t_end=7200.0)
+    sc.flags |= S.E7_Y_ORIENT
+    return sc
+
+
+def device_sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def time_precision(sc, prec, steps, warmup, stage_samples):
+    from mfcm_amd import mfsv
+    cat = mfsv.Catchment(sc, precision=prec)
+    cat.run_begin()
+    if warmup:
+        cat.run(warmup, graph=True)
+    device_sync()
+    s0 = cat.stats()
+    t0 = time.perf_counter()
+    cat.run(steps, graph=True)
+    device_sync()
+    wall = time.perf_counter() - t0
+    ev_ms = cat.last_run_ms()
+    s1 = cat.stats()
+    acc = s1["accepted"] - s0["accepted"]
+    rej = s1["rejected"] - s0["rejected"]
+    # per-launch stage-kernel durations from HIP events (eager launches)
+    cat.run(stage_samples, graph=2)
+    st_ms, launches = cat.last_stage_ms()
+    s2 = cat.stats()
+    out = {
+        "wall_s": wall, "event_ms": ev_ms, "accepted": acc, "rejected": rej,
+        "attempted": acc + rej, "cell_updates": acc * sc.ncells,
+        "cell_updates_per_s": acc * sc.ncells / wall,
+        "stage_ms_mean": st_ms / max(launches, 1), "stage_launches": launches,
+        "sim_time_s": s1["time"], "done": s2["done"],
+        "device_bytes": cat.device_bytes(),
+    }
+    cat.close()
+    return out
+
+
+def accuracy(sc0, horizon):
+    """FP32 form vs FP64 plain after `horizon` simulated seconds."""
+    import copy
+
+    from mfcm_amd import mfsv
+    sc = copy.copy(sc0)
+    sc.t_end = horizon
+    res = {}
+    for name, prec in (("f64", mfsv.F64), ("f32", mfsv.F32)):
+        cat = mfsv.Catchment(sc, precision=prec)
+        cat.run_begin()
+        st = cat.stats()
+        while not st["done"]:
+            st = cat.run(4000, graph=True)
+            log(f"  accuracy {name}: t = {st['time']:.1f} s, {st['accepted']} accepted, "
+                f"{st['rejected']} rejected")
+        h, u, v = cat.state()
+        res[name] = (h, u, v, st)
+        cat.close()
+    h64, u64, v64, st64 = res["f64"]
+    h32, u32, v32, st32 = res["f32"]
+    wet = h64 > 1e-3
+    vel_err = float(max(np.abs(u32 - u64)[wet].max(initial=0.0), np.abs(v32 - v64)[wet].max(initial=0.0)))
+    V64 = float(np.sum(h64)) * sc.dx * sc.dy
+    V32 = float(np.sum(h32)) * sc.dx * sc.dy
+    mb = lambda s: (s["volume"] - s["volume0"] - s["source_volume"]) / max(abs(s["volume"]), 1e-300)
+    return {
+        "horizon_s": horizon,
+        "max_depth_err_m": float(np.abs(h32 - h64).max()),
+        "max_velocity_err_m_s_wet": vel_err,
+        "rel_mass_err_vs_f64": abs(V32 - V64) / max(V64, 1e-300),
+        "mass_balance_residual_f64": mb(st64),
+        "mass_balance_residual_f32": mb(st32),
+        "volume_m3_f64": V64,
+        "timesteps_f64": st64["accepted"] + st64["rejected"],
+        "timesteps_f32": st32["accepted"] + st32["rejected"],
+    }
+
+
+def cpu_baseline(sc, budget_s):
+    """The CPU oracle on the first timesteps of the same workload."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sim = O.sim_from_scenario(sc)
+    sim.start_new_step(0.0)
+    dt, t_local = sc.initial_dt, 0.0
+    n_acc = n_att = 0
+    t0 = time.perf_counter()
+    while True:
+        new_dt, rep = sim.do_timestep(t_local, dt)
+        n_att += 1
+        if not rep:
+            sim.accept()
+            t_local += dt
+            n_acc += 1
+        dt = new_dt
+        if t_local + dt > sc.step_duration:
+            dt = sc.step_duration - t_local
+        elif t_local + 2 * dt > sc.step_duration:
+            dt = (sc.step_duration - t_local) * 0.4
+        el = time.perf_counter() - t0
+        if (el > budget_s and n_acc >= 2) or t_local >= sc.step_duration or n_att >= 10000:
+            break
+    return {
+        "value": n_acc * sc.ncells / el,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {n_att} timesteps ({n_acc} accepted) of the same {sc.nx}x{sc.ny} "
+                  f"workload from the dry start, FP64 C oracle with OpenMP over rows, "
+                  f"{el:.1f} s of CPU time",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000, help="timed timesteps")
+    ap.add_argument("--warmup", type=int, default=200, help="untimed timesteps")
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--stage-samples", type=int, default=200)
+    ap.add_argument("--accuracy-horizon", type=float, default=300.0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world}); the catchment "
+              f"would shard into row strips with a 2-row halo per RK stage.", file=sys.stderr)
+        sys.exit(2)
+
+    # torch first: its HIP runtime must be the one in the process before
+    # libmfsv_hip.so binds libamdhip64.so.7 (two runtimes cannot share a GPU).
+    import torch
+    torch.cuda.init()
+    from mfcm_amd import mfsv
+    mfsv.lib()
+    if mfsv.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+
+    log(f"generating scenario {args.size}x{args.size}")
+    sc = make_scenario(args.size)
+    N = sc.ncells
+    res = {}
+    for name, prec in (("f32", mfsv.F32), ("f64", mfsv.F64)):
+        log(f"timing {name}: warmup {args.warmup}, steps {args.steps}")
+        res[name] = time_precision(sc, prec, args.steps, args.warmup, args.stage_samples)
+        log(f"  {name}: {res[name]['cell_updates_per_s']:.4g} cell-updates/s, "
+            f"stage {res[name]['stage_ms_mean'] * 1e3:.1f} us")
+    acc = None
+    if not args.no_accuracy:
+        log(f"accuracy FP32 vs FP64 over {args.accuracy_horizon} s")
+        acc = accuracy(sc, args.accuracy_horizon)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(sc, args.cpu_budget)
+
+    B = {"f64": 8 * (15 + 2 * 2), "f32": 4 * (15 + 2 * 2)}
+    roof = {}
+    for name in ("f32", "f64"):
+        r = res[name]
+        ach = (B[name] / 2.0) * N / (r["stage_ms_mean"] * 1e-3) / 1e9
+        roof[name] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                      "kernel": f"mfsv::k_stage<{'float' if name == 'f32' else 'double'}>",
+                      "stage_us_mean": r["stage_ms_mean"] * 1e3,
+                      "bytes_per_cell_update": B[name]}
+    f32, f64 = res["f32"], res["f64"]
+    line = {
+        "metric": METRIC,
+        "value": f32["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f32["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 time, controller and mass)",
+        "data": "synthetic",
+        "config": {"workload": f"config1 single catchment {args.size}x{args.size} dx=2m, "
+                               f"30 hills + fBm 1m on +300m, Voronoi n, drifting storm, "
+                               f"heun2, CFL 0.5, flags E1|E2|E4|E7|E8|E9, dry start",
+                   "cells": N, "precision_form": "f32"},
+        "roofline": roof["f32"],
+        "cpu_baseline": cpu,
+        "fp64_plain": {"value": f64["cell_updates_per_s"], "unit": "cell-updates/s",
+                       "ms_per_step": f64["wall_s"] * 1e3 / args.steps,
+                       "roofline": roof["f64"]},
+        "f32_over_f64": f32["cell_updates_per_s"] / f64["cell_updates_per_s"],
+        "timesteps": {k: {"accepted": res[k]["accepted"], "rejected": res[k]["rejected"],
+                          "sim_time_s": res[k]["sim_time_s"]} for k in res},
+        "accuracy": acc,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -40,6 +40,10 @@ extern "C" {
 #define MFSV_E2_WEST_OUTFLOW 0x02 /* transmissive outflow on the x = 0 edge */
 #define MFSV_E4_STORM 0x08        /* analytic drifting Gaussian storm        */
 #define MFSV_E7_Y_ORIENT 0x40     /* y slope taken south->north             */
+#define MFSV_E8_DRY_VELOCITY 0x80 /* completed stage states: u = v = 0
+                                     where h <= h_dry                        */
+#define MFSV_E9_VELOCITY_CAP 0x100 /* completed stage states: |u|, |v| <=
+                                      u_max                                 */
 
 /* Cell fields (Field/DataArray containers of the reference, one value per
  * cell, row-major i = y*nx + x as Mesh/Cartesian2DMesh.hpp:308-309). */
@@ -86,6 +90,8 @@ typedef struct mfsv_params {
   int64_t head_x0, head_x1;
   /* E4 storm: rate = peak*exp(-r^2/(2 sigma^2)) around (cx0+vx t, cy0+vy t). */
   double storm_peak, storm_sigma, storm_cx0, storm_cy0, storm_vx, storm_vy;
+  double h_dry; /* E8 dry depth (m)          */
+  double u_max; /* E9 velocity cap (m/s)     */
 } mfsv_params;
 
 typedef struct mfsv_stats {
@@ -162,14 +168,19 @@ int mfsv_volume(mfsv_ctx* ctx, double* volume);
 /* Reset to outer step 0 with dt = initial dt (forcing table required when a
  * scalar discharge or head boundary is configured). */
 int mfsv_run_begin(mfsv_ctx* ctx);
-/* Attempt up to max_timesteps timesteps (fewer if the run finishes);
- * graph != 0 replays hipGraph-captured timesteps. */
+/* Attempt up to max_timesteps timesteps (fewer if the run finishes).
+ * graph: 0 = launch each kernel; 1 = replay hipGraph-captured batches of
+ * timesteps; 2 = launch each kernel and bracket every timestep's stage
+ * kernels with HIP events (read with mfsv_last_stage_ms). */
 int mfsv_run(mfsv_ctx* ctx, int64_t max_timesteps, int graph);
 int mfsv_get_stats(mfsv_ctx* ctx, mfsv_stats* out);
 
 /* Timing of the last mfsv_run: milliseconds measured with HIP events on the
  * context's stream around the launches of that call. */
 int mfsv_last_run_ms(mfsv_ctx* ctx, double* ms);
+/* Stage-kernel time of the last mfsv_run with graph == 2: the summed HIP
+ * event intervals around the stage launches, and the number of launches. */
+int mfsv_last_stage_ms(mfsv_ctx* ctx, double* ms, int64_t* launches);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,10 @@ struct mfsv_ctx {
   hipGraph_t graph = nullptr;
   int64_t graph_steps = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> sev;  // per-timestep stage brackets (timed runs)
   double last_ms = 0.0;
+  double stage_ms = 0.0;
+  int64_t stage_launches = 0;
   double volume0 = 0.0;
   std::string err;
   int64_t bytes = 0;
@@ -189,6 +192,7 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches) {
   A.hb0f = tp<T>(c->hb0f); A.hb1f = tp<T>(c->hb1f);
   A.n_sh = T(c->P.n_shallow); A.n_dp = T(c->P.n_deep);
   A.d_sh = T(c->P.d_shallow); A.d_dp = T(c->P.d_deep);
+  A.h_dry = T(c->P.h_dry); A.u_max = T(c->P.u_max);
   A.storm[0] = c->P.storm_peak; A.storm[1] = c->P.storm_sigma;
   A.storm[2] = c->P.storm_cx0; A.storm[3] = c->P.storm_cy0;
   A.storm[4] = c->P.storm_vx; A.storm[5] = c->P.storm_vy;
@@ -399,6 +403,8 @@ int mfsv_destroy(mfsv_ctx* c) {
   if (c->hctrl) (void)hipHostFree(c->hctrl);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (auto e : c->sev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MFSV_OK;
@@ -671,20 +677,40 @@ int mfsv_run(mfsv_ctx* c, int64_t max_timesteps, int use_graph) {
   if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
   int rc;
   const int64_t batch = 32;
-  if (use_graph && c->graph_steps != batch)
+  const bool timed = use_graph == 2;
+  if (use_graph == 1 && c->graph_steps != batch)
     if ((rc = build_graph(c, batch))) return rc;
+  if (timed && c->sev.empty()) {
+    c->sev.resize(2 * batch, nullptr);
+    for (auto& e : c->sev) HIPCHK(c, hipEventCreate(&e));
+  }
+  c->stage_ms = 0.0;
+  c->stage_launches = 0;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   int64_t left = max_timesteps;
   while (left > 0) {
     const int64_t n = left < batch ? left : batch;
-    if (use_graph && n == batch) {
+    if (use_graph == 1 && n == batch) {
       HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     } else {
-      for (int64_t k = 0; k < n; ++k)
-        if ((rc = launch_timestep(c, 0))) return rc;
+      for (int64_t k = 0; k < n; ++k) {
+        if (timed) HIPCHK(c, hipEventRecord(c->sev[2 * k], c->stream));
+        for (int s = 0; s < c->nstages; ++s)
+          if ((rc = launch_stage(c, s, false, false))) return rc;
+        if (timed) HIPCHK(c, hipEventRecord(c->sev[2 * k + 1], c->stream));
+        if ((rc = launch_ctrl(c, 0))) return rc;
+      }
     }
     left -= n;
     if ((rc = read_ctrl(c))) return rc;
+    if (timed) {
+      for (int64_t k = 0; k < n; ++k) {
+        float m = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&m, c->sev[2 * k], c->sev[2 * k + 1]));
+        c->stage_ms += m;
+      }
+      c->stage_launches += n * c->nstages;
+    }
     if (c->hctrl->done) break;
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -722,6 +748,13 @@ int mfsv_get_stats(mfsv_ctx* c, mfsv_stats* s) {
 int mfsv_last_run_ms(mfsv_ctx* c, double* ms) {
   if (!c || !ms) return MFSV_ERR_ARG;
   *ms = c->last_ms;
+  return MFSV_OK;
+}
+
+int mfsv_last_stage_ms(mfsv_ctx* c, double* ms, int64_t* launches) {
+  if (!c || !ms || !launches) return MFSV_ERR_ARG;
+  *ms = c->stage_ms;
+  *launches = c->stage_launches;
   return MFSV_OK;
 }
 

@@ -78,6 +78,7 @@ struct StageArgs {
   const T* nfield;
   const T* q0f; const T* q1f; const T* hb0f; const T* hb1f;
   T n_sh, n_dp, d_sh, d_dp;
+  T h_dry, u_max;           // E8 dry depth, E9 velocity cap
   double storm[6];          // peak, sigma, cx0, cy0, vx, vy
   // accumulators: dst[j]; null = the new-state buffer U[cur^1]
   T* acc_h[kMaxStages]; T* acc_u[kMaxStages]; T* acc_v[kMaxStages];
@@ -346,6 +347,13 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArg
             oh = bh + dhdt * sc[j];
             ou = bu + dudt * sc[j];
             ov = bv + dvdt * sc[j];
+            if (j == 0) {  // row s+1 is complete after this stage
+              if ((A.flags & 0x80) && oh <= A.h_dry) { ou = T(0); ov = T(0); }  // E8
+              if (A.flags & 0x100) {                                           // E9
+                ou = fmin_t(fmax_t(ou, -A.u_max), A.u_max);
+                ov = fmin_t(fmax_t(ov, -A.u_max), A.u_max);
+              }
+            }
             dh_[i] = oh; du_[i] = ou; dv_[i] = ov;
           }
         }

@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(HERE, "libmfsv_hip.so")
 
 F64, F32 = 0, 1
 E1_RAIN_GUARD, E2_WEST_OUTFLOW, E4_STORM, E7_Y_ORIENT = 0x01, 0x02, 0x08, 0x40
+E8_DRY_VELOCITY, E9_VELOCITY_CAP = 0x80, 0x100
 FIELD = {"h": 0, "u": 1, "v": 2, "zb": 3, "manning_n": 4, "qbdy0": 5, "qbdy1": 6,
          "hbdy0": 7, "hbdy1": 8, "dhdt": 9, "dudt": 10, "dvdt": 11,
          "branch_x": 12, "branch_y": 13}
@@ -70,7 +71,8 @@ class Params(C.Structure):
                 ("head_x0", C.c_int64), ("head_x1", C.c_int64),
                 ("storm_peak", C.c_double), ("storm_sigma", C.c_double),
                 ("storm_cx0", C.c_double), ("storm_cy0", C.c_double),
-                ("storm_vx", C.c_double), ("storm_vy", C.c_double)]
+                ("storm_vx", C.c_double), ("storm_vy", C.c_double),
+                ("h_dry", C.c_double), ("u_max", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -109,6 +111,7 @@ SIGNATURES = {
     "mfsv_run": (_I, [_P, _I64, _I]),
     "mfsv_get_stats": (_I, [_P, C.POINTER(Stats)]),
     "mfsv_last_run_ms": (_I, [_P, C.POINTER(C.c_double)]),
+    "mfsv_last_stage_ms": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
 }
 
 _lib = None
@@ -205,6 +208,8 @@ class Catchment:
             st = sc.storm
             p.storm_peak, p.storm_sigma = st.peak, st.sigma
             p.storm_cx0, p.storm_cy0, p.storm_vx, p.storm_vy = st.cx0, st.cy0, st.vx, st.vy
+        p.h_dry = float(getattr(sc, "h_dry", 1e-3))
+        p.u_max = float(getattr(sc, "u_max", 10.0))
         self.params = p
         self._check(lib().mfsv_set_params(self._h, C.byref(p)))
         self._check(lib().mfsv_set_time(self._h, 0.0, float(sc.t_end), float(sc.step_duration),
@@ -352,8 +357,16 @@ class Catchment:
         self._check(lib().mfsv_run_begin(self._h))
 
     def run(self, max_timesteps=1 << 40, graph=True):
-        self._check(lib().mfsv_run(self._h, int(max_timesteps), 1 if graph else 0))
+        """graph: True/1 hipGraph replay, False/0 eager, 2 eager with
+        per-timestep HIP events around the stage kernels."""
+        mode = int(graph) if not isinstance(graph, bool) else (1 if graph else 0)
+        self._check(lib().mfsv_run(self._h, int(max_timesteps), mode))
         return self.stats()
+
+    def last_stage_ms(self):
+        m, n = C.c_double(), C.c_int64()
+        self._check(lib().mfsv_last_stage_ms(self._h, C.byref(m), C.byref(n)))
+        return m.value, n.value
 
     def stats(self):
         s = Stats()

@@ -129,6 +129,8 @@ class Scenario:
     head_cols: int = 0                   # parity-mode outflow: head(0) on x < head_cols
     flags: int = 0
     cn_target: float = 1.0
+    h_dry: float = 1e-3                  # E8: no velocity where h <= h_dry
+    u_max: float = 10.0                  # E9: velocity components within +-u_max
     t_end: float = 3600.0
     step_duration: float = 60.0
     initial_dt: float = 1.0
@@ -152,6 +154,11 @@ E1_RAIN_GUARD = 0x01
 E2_WEST_OUTFLOW = 0x02
 E4_STORM = 0x08
 E7_Y_ORIENT = 0x40
+E8_DRY_VELOCITY = 0x80
+E9_VELOCITY_CAP = 0x100
+# physics mode: rain on dry cells, free outflow, y orientation fixed, dry
+# cells carry no velocity, velocity components capped
+PHYSICS = 0x01 | 0x02 | 0x40 | 0x80 | 0x100
 
 
 def rain_series(rate_m_s: float, duration_s: float, t_end: float, step: float):
@@ -175,7 +182,7 @@ def config0(seed=0, n=256, dx=5.0, prewet=0.0, physics=False, t_end=3600.0):
     sc = Scenario("config0", n, n, dx, dx, zb, h0, z.copy(), z.copy(), 0.03,
                   rain_t=rt, rain_v=rv, t_end=t_end)
     if physics:
-        sc.flags = E1_RAIN_GUARD | E2_WEST_OUTFLOW
+        sc.flags = PHYSICS
         sc.cn_target = 0.5
     else:
         sc.head_cols = 1
@@ -192,7 +199,7 @@ def config1(seed=1, n=2048, dx=2.0, t_end=7200.0, prewet=0.0):
     z = np.zeros((n, n))
     sc = Scenario("config1", n, n, dx, dx, zb, np.full((n, n), float(prewet)), z.copy(),
                   z.copy(), nm, storm=st, t_end=t_end,
-                  flags=E1_RAIN_GUARD | E2_WEST_OUTFLOW | E4_STORM, cn_target=0.5)
+                  flags=PHYSICS | E4_STORM, cn_target=0.5)
     return sc
 
 
@@ -210,7 +217,7 @@ def config2(seed=2, n=4096, dx=1.0, t_end=600.0, wall_x=1000, breach=50):
     h0[:, :wall_x] = 5.0
     z = np.zeros((n, n))
     return Scenario("config2", n, n, dx, dx, np.ascontiguousarray(zb), h0, z.copy(), z.copy(),
-                    0.025, t_end=t_end, cn_target=0.5)
+                    0.025, t_end=t_end, cn_target=0.5, flags=E7_Y_ORIENT | E8_DRY_VELOCITY)
 
 
 def ensemble_member(seed: int, side_lo=128, side_hi=1024, t_end=3600.0):
@@ -229,7 +236,7 @@ def ensemble_member(seed: int, side_lo=128, side_hi=1024, t_end=3600.0):
     rt, rv = rain_series(rate, dur, t_end, 60.0)
     return Scenario(f"member{seed}", nx, ny, dx, dx, zb, z.copy(), z.copy(), z.copy(), n_man,
                     rain_t=rt, rain_v=rv, t_end=t_end,
-                    flags=E1_RAIN_GUARD | E2_WEST_OUTFLOW, cn_target=0.5)
+                    flags=PHYSICS, cn_target=0.5)
 
 
 def dam_break_bump(nx=64, ny=48, dx=5.0, dy=5.0, seed=3, h_up=2.0):

@@ -23,6 +23,8 @@ E1_RAIN_GUARD = 0x01
 E2_WEST_OUTFLOW = 0x02
 E4_STORM = 0x08
 E7_Y_ORIENT = 0x40
+E8_DRY_VELOCITY = 0x80
+E9_VELOCITY_CAP = 0x100
 
 # Named Butcher tables (TemporalScheme/RungeKutta.cpp:54-116), rows as the
 # reference's a_ vectors.
@@ -99,6 +101,8 @@ def lib():
         L.or_sim_add_head.argtypes = [p, C.c_int, p, p, p]
         L.or_sim_set_storm.argtypes = [p, C.POINTER(Storm), d, d]
         L.or_sim_set_time.argtypes = [p, d, d, d, d]
+        L.or_sim_set_dry_depth.argtypes = [p, d]
+        L.or_sim_set_velocity_cap.argtypes = [p, d]
         L.or_sim_num_steps.restype = i64
         L.or_sim_num_steps.argtypes = [p]
         L.or_sim_start_new_step.argtypes = [p, d]
@@ -245,6 +249,12 @@ class Sim:
     def set_time(self, start, end, step_duration=60.0, initial_dt=1.0):
         lib().or_sim_set_time(self._h, start, end, step_duration, initial_dt)
 
+    def set_velocity_cap(self, u_max):
+        lib().or_sim_set_velocity_cap(self._h, float(u_max))
+
+    def set_dry_depth(self, h_dry):
+        lib().or_sim_set_dry_depth(self._h, float(h_dry))
+
     def num_steps(self):
         return lib().or_sim_num_steps(self._h)
 
@@ -326,4 +336,6 @@ def sim_from_scenario(sc):
     if sc.storm is not None:
         sim.set_storm(sc.storm.as_dict())
     sim.set_time(0.0, sc.t_end, sc.step_duration, sc.initial_dt)
+    sim.set_dry_depth(sc.h_dry)
+    sim.set_velocity_cap(sc.u_max)
     return sim

@@ -266,6 +266,8 @@ struct or_sim {
   double dx, dy;
   int flags;
   double cn_target;
+  double h_dry;                             /* E8 dry depth */
+  double u_max;                             /* E9 velocity cap */
   int ns;                                   /* RK stages */
   double a[OR_MAX_STAGES + 1][OR_MAX_STAGES + 1];
   double *zb, *dzbdx, *dzbdy;               /* Constants.cpp:27-44 */
@@ -665,7 +667,26 @@ void or_sim_update_y(or_sim* S, int step, double dt) {
       yv[c] += kv[c] * s;
     }
   }
+  /* E8: a completed stage state carries no velocity where h <= h_dry. */
+  if (S->flags & OR_E8_DRY_VELOCITY) {
+    const double hd = S->h_dry;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < S->nc; ++c)
+      if (yh[c] <= hd) { yu[c] = 0.0; yv[c] = 0.0; }
+  }
+  /* E9: each velocity component of a completed stage state within +-u_max. */
+  if (S->flags & OR_E9_VELOCITY_CAP) {
+    const double um = S->u_max;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < S->nc; ++c) {
+      yu[c] = fmin(fmax(yu[c], -um), um);
+      yv[c] = fmin(fmax(yv[c], -um), um);
+    }
+  }
 }
+
+void or_sim_set_dry_depth(or_sim* S, double h_dry) { S->h_dry = h_dry; }
+void or_sim_set_velocity_cap(or_sim* S, double u_max) { S->u_max = u_max; }
 
 /* State.cpp:199-229: max over cells of dt ((|u| + c)/dx + (|v| + c)/dy). */
 double or_sim_control_number(or_sim* S, int s, double timestep) {

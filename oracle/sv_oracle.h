@@ -48,6 +48,8 @@ extern "C" {
 #define OR_E2_WEST_OUTFLOW 0x02 /* transmissive ghost on the x = 0 edge faces */
 #define OR_E4_STORM        0x08 /* analytic drifting Gaussian storm rainfall */
 #define OR_E7_Y_ORIENT     0x40 /* y slope taken south->north (physical sign) */
+#define OR_E8_DRY_VELOCITY 0x80 /* no velocity in completed stage states with h <= h_dry */
+#define OR_E9_VELOCITY_CAP 0x100 /* |u|, |v| <= u_max in completed stage states */
 
 /* Face branch codes written to the branch face field (FluxKernel.cpp:206-293). */
 #define OR_BRANCH_WET     1.0
@@ -106,6 +108,8 @@ void or_sim_set_storm(or_sim* S, const or_storm* st, double origin_x,
                       double origin_y);
 void or_sim_set_time(or_sim* S, double start, double end, double step_duration,
                      double initial_dt);
+void or_sim_set_dry_depth(or_sim* S, double h_dry);
+void or_sim_set_velocity_cap(or_sim* S, double u_max);
 int64_t or_sim_num_steps(const or_sim* S);
 
 void or_sim_start_new_step(or_sim* S, double time_now);

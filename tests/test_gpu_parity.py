@@ -90,7 +90,7 @@ def _compare_states(cat, sim, tol=1e-9):
 
 
 @pytest.mark.parametrize("case", ["dambreak_parity", "dambreak_e7", "config0_prewet_parity",
-                                  "config0_physics"])
+                                  "config0_physics", "config1_physics"])
 def test_run_matches_oracle(gpu_lib, case):
     from mfcm_amd.mfsv import Catchment
     if case.startswith("dambreak"):
@@ -98,6 +98,8 @@ def test_run_matches_oracle(gpu_lib, case):
         sc.t_end = 60.0 if case.endswith("parity") else 120.0
         if case.endswith("e7"):
             sc.flags |= S.E7_Y_ORIENT
+    elif case == "config1_physics":
+        sc = S.config1(seed=1, n=64, dx=2.0, t_end=180.0)
     elif case == "config0_prewet_parity":
         sc = S.config0(seed=0, n=64, dx=5.0, prewet=0.002, t_end=300.0)
     else:
