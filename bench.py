@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB of a plain device copy to run first (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -195,6 +197,16 @@ def main():
     if mfsv.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
+
+    if args.copy_probe:
+        # A plain copy of known size: the calibration kernel for HBM counters.
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 5 x {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
 
     log(f"generating scenario {args.size}x{args.size}")
     sc = make_scenario(args.size)
