@@ -7,6 +7,7 @@
 // number, mfsv_run checks the finished flag once per batch of timesteps).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -15,7 +16,7 @@
 #include <vector>
 
 #include "../../include/mfsv.h"
-#include "sv_stage.hpp"
+#include "sv_ctrl.hpp"
 
 using namespace mfsv;
 
@@ -53,6 +54,7 @@ struct mfsv_ctx {
   int64_t nstrips = 0, nrb = 0, ntiles = 0, nblocks = 0;
   int cur = 0;
   bool nfield_set = false;
+  bool nanbed = false;  // the bed has coded-out (NaN) cells
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int64_t graph_steps = 0;
@@ -163,24 +165,26 @@ template <typename T>
 T* tp(const Buf& b) { return static_cast<T*>(b.p); }
 
 template <typename T>
-StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches) {
+StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc) {
   StageArgs<T> A{};
-  A.nx = c->nx; A.ny = c->ny;
-  A.nrb = c->nrb; A.ntiles = c->ntiles; A.rows = c->rows;
+  A.nx = (int32_t)c->nx; A.ny = (int32_t)c->ny;
+  A.nrb = (int32_t)c->nrb; A.ntiles = (int32_t)c->ntiles; A.rows = c->rows;
   A.stage = s; A.nstages = c->nstages;
   A.last = (!kout && s == c->nstages - 1) ? 1 : 0;
-  A.kout = kout ? 1 : 0;
   A.flags = c->P.flags;
+  A.nanbed = c->nanbed ? 1 : 0;
   A.manning = c->P.manning;
   A.discharge = c->P.discharge;
   A.head = c->P.head;
-  A.hx0 = c->P.head_x0; A.hx1 = c->P.head_x1;
+  A.hx0 = (int32_t)std::max<int64_t>(0, std::min<int64_t>(c->P.head_x0, c->nx));
+  A.hx1 = (int32_t)std::max<int64_t>(0, std::min<int64_t>(c->P.head_x1, c->nx));
   A.dx = T(c->dx); A.dy = T(c->dy);
   A.rdx = T(1.0 / c->dx); A.rdy = T(1.0 / c->dy);
   A.hdx = T(0.5 * c->dx); A.hdy = T(0.5 * c->dy);
-  A.dxd = c->dx; A.dyd = c->dy; A.area = c->dx * c->dy;
+  A.dxd = c->dx; A.dyd = c->dy;
   A.ox = c->ox; A.oy = c->oy;
   A.c_s = c->tab[s][0];
+  A.rstep = 1.0 / c->step_dur;
   A.Uh[0] = tp<T>(c->U[0][0]); A.Uu[0] = tp<T>(c->U[0][1]); A.Uv[0] = tp<T>(c->U[0][2]);
   A.Uh[1] = tp<T>(c->U[1][0]); A.Uu[1] = tp<T>(c->U[1][1]); A.Uv[1] = tp<T>(c->U[1][2]);
   if (s > 0) {  // stage input: accumulator row s
@@ -193,16 +197,15 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches) {
   A.n_sh = T(c->P.n_shallow); A.n_dp = T(c->P.n_deep);
   A.d_sh = T(c->P.d_shallow); A.d_dp = T(c->P.d_deep);
   A.h_dry = T(c->P.h_dry); A.u_max = T(c->P.u_max);
-  A.storm[0] = c->P.storm_peak; A.storm[1] = c->P.storm_sigma;
-  A.storm[2] = c->P.storm_cx0; A.storm[3] = c->P.storm_cy0;
-  A.storm[4] = c->P.storm_vx; A.storm[5] = c->P.storm_vy;
+  A.storm = {c->P.storm_peak, c->P.storm_sigma, c->P.storm_cx0, c->P.storm_cy0,
+             c->P.storm_vx, c->P.storm_vy};
+  *nacc = 0;
   if (kout) {
-    A.nacc = 0;
     A.k_h = tp<T>(c->kf[0]); A.k_u = tp<T>(c->kf[1]); A.k_v = tp<T>(c->kf[2]);
   } else {
     // rows i = s+1 .. n receive k_s * a(i, s+1)  (RungeKutta.hpp:377-388)
-    A.nacc = c->nstages - s;
-    for (int j = 0; j < A.nacc; ++j) {
+    *nacc = c->nstages - s;
+    for (int j = 0; j < *nacc; ++j) {
       const int i = s + 1 + j;
       A.acoef[j] = c->tab[i][s + 1];
       if (i < c->nstages) {
@@ -218,13 +221,36 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches) {
   return A;
 }
 
-int launch_stage(mfsv_ctx* c, int s, bool kout, bool branches) {
+template <typename T, bool QUICK>
+void launch_q(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
   dim3 grid((unsigned)c->nblocks), block(kWave * kWavesPerBlock);
-  if (c->prec == MFSV_F64) {
-    hipLaunchKernelGGL(k_stage<double>, grid, block, 0, c->stream, stage_args<double>(c, s, kout, branches));
-  } else {
-    hipLaunchKernelGGL(k_stage<float>, grid, block, 0, c->stream, stage_args<float>(c, s, kout, branches));
+  switch (kout ? 0 : nacc) {
+    case 0: hipLaunchKernelGGL((k_stage<T, 0, true, QUICK>), grid, block, 0, c->stream, A); break;
+    case 1: hipLaunchKernelGGL((k_stage<T, 1, false, QUICK>), grid, block, 0, c->stream, A); break;
+    case 2: hipLaunchKernelGGL((k_stage<T, 2, false, QUICK>), grid, block, 0, c->stream, A); break;
+    case 3: hipLaunchKernelGGL((k_stage<T, 3, false, QUICK>), grid, block, 0, c->stream, A); break;
+    default: hipLaunchKernelGGL((k_stage<T, 4, false, QUICK>), grid, block, 0, c->stream, A); break;
   }
+}
+
+// True when x is a power of two: x/d == x * (1/d) exactly.
+bool pow2(double d) {
+  int e;
+  return std::frexp(d, &e) == 0.5;
+}
+
+template <typename T>
+void launch_typed(mfsv_ctx* c, int s, bool kout, bool branches) {
+  int nacc = 0;
+  const StageArgs<T> A = stage_args<T>(c, s, kout, branches, &nacc);
+  const bool quick = sizeof(T) == 4 || (pow2(c->dx) && pow2(c->dy));
+  if (quick) launch_q<T, true>(c, A, nacc, kout);
+  else launch_q<T, false>(c, A, nacc, kout);
+}
+
+int launch_stage(mfsv_ctx* c, int s, bool kout, bool branches) {
+  if (c->prec == MFSV_F64) launch_typed<double>(c, s, kout, branches);
+  else launch_typed<float>(c, s, kout, branches);
   HIPCHK(c, hipGetLastError());
   return MFSV_OK;
 }
@@ -333,6 +359,8 @@ int mfsv_device_count(int* count) {
 
 int mfsv_create(const mfsv_grid* g, int precision, int device, mfsv_ctx** out) {
   if (!g || !out || g->nx < 1 || g->ny < 1 || !(g->dx > 0) || !(g->dy > 0)) return MFSV_ERR_ARG;
+  if (g->nx > (1 << 30) || g->ny > (1 << 30) || g->nx * g->ny >= (int64_t(1) << 31))
+    return MFSV_ERR_ARG;  // 32-bit cell indices in the kernels
   if (precision != MFSV_F64 && precision != MFSV_F32) return MFSV_ERR_ARG;
   *out = nullptr;
   mfsv_ctx* c = new (std::nothrow) mfsv_ctx();
@@ -460,6 +488,21 @@ int mfsv_set_scheme(mfsv_ctx* c, int nstages, const double* tab) {
   return rc;
 }
 
+int mfsv_set_tile_rows(mfsv_ctx* c, int rows) {
+  if (!c || rows < 1 || rows > 256) return MFSV_ERR_ARG;
+  if (rows == c->rows) return MFSV_OK;
+  drop_graph(c);
+  c->rows = rows;
+  geometry(c);
+  return ensure_partials(c);
+}
+
+int mfsv_get_tile_rows(const mfsv_ctx* c, int* rows) {
+  if (!c || !rows) return MFSV_ERR_ARG;
+  *rows = c->rows;
+  return MFSV_OK;
+}
+
 int mfsv_set_time(mfsv_ctx* c, double start, double end, double step_duration, double initial_dt) {
   if (!c || !(step_duration > 0) || !(initial_dt > 0) || !(end >= start)) return MFSV_ERR_ARG;
   c->start = start; c->end = end; c->step_dur = step_duration; c->dt0 = initial_dt;
@@ -512,6 +555,9 @@ int mfsv_upload(mfsv_ctx* c, int field, const double* host, int64_t n) {
         off = std::isfinite(m) ? m : 0.0;
       }
       c->zoff = off;
+      c->nanbed = false;
+      for (int64_t i = 0; i < n && !c->nanbed; ++i) c->nanbed = std::isnan(host[i]);
+      drop_graph(c);
       return put(c, c->zb, host, n, off);
     }
     case MFSV_FIELD_MANNING_N: {

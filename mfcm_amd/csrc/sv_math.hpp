@@ -69,6 +69,32 @@ __device__ __forceinline__ double pow43(double x) {
   const double c = x * r;
   return c * c;
 }
+// exp(x) for x <= 0 from IEEE operations only, the same expression as the
+// CPU oracle's or_exp_neg (Cody-Waite reduction by ln 2, degree-12 Taylor
+// polynomial, exact 2^k scaling): within 2.3 ulp, identical bits on both.
+__device__ __forceinline__ double exp_neg(double x) {
+  if (x < -708.0) return 0.0;
+  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  const double log2e = 1.44269504088896338700e+00;
+  const double k = floor(x * log2e + 0.5);
+  const double r = (x - k * ln2_hi) - k * ln2_lo;
+  double p = 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)k);
+}
+__device__ __forceinline__ float exp_neg(float x) { return __expf(x); }
+
 // FP32 form: hardware log2/exp2.
 __device__ __forceinline__ float pow43(float x) {
   return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (4.0f / 3.0f));
@@ -225,6 +251,123 @@ __device__ __forceinline__ Flux<T> face_flux(Side<T> L, Side<T> R, int edge, T h
 zero:                                        // :47-60
   F.h = T(0); F.n = T(0); F.t = T(0); F.z = T(0);
   *branch = 0.0f;
+  return F;
+}
+
+// ---------------------------------------------------------------------------
+// Fast forms used when the catchment has no coded-out (NaN-bed) cell.  They
+// return the same values as minmod3 / face_flux (signed zeros aside) with far
+// fewer instructions.
+
+__device__ __forceinline__ double copysign_t(double m, double s) { return __builtin_copysign(m, s); }
+__device__ __forceinline__ float copysign_t(float m, float s) { return __builtin_copysignf(m, s); }
+
+// a*b + c: unfused in FP64 (the reference rounds the product), fused in the
+// FP32 form.
+__device__ __forceinline__ double mad_t(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float mad_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Square root: correctly rounded in FP64, the hardware v_sqrt_f32 in FP32.
+__device__ __forceinline__ double fsqrt_t(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float fsqrt_t(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+// Minmod3 from the three quotients ql = (Uc-Ul)/d, qr = (Ur-Uc)/d and
+// qc = (Ur-Ul)/d (no NaN among them).  With theta = 2 the reference's
+// lhs = 2 ql, rhs = 2 qr, central = qc/2 exactly; minmod of those is
+// sign * min(2|ql|, 2|qr|, |qc|/2) when ql and qr share a sign (then qc
+// does too), else 0.  One-sided at an edge: the reference reduces to the
+// remaining quotient.
+template <typename T>
+__device__ __forceinline__ T minmod_q(T ql, T qr, T qc, bool hasl, bool hasr) {
+  const bool same = (ql > T(0) && qr > T(0)) || (ql < T(0) && qr < T(0));
+  // min(2|ql|, 2|qr|, |qc|/2) == 2 min(|ql|, |qr|, |qc|/4): power-of-two scalings
+  const T m = T(2) * fmin_t(fmin_t(fabs_t(ql), fabs_t(qr)), T(0.25) * fabs_t(qc));
+  const T in = same ? copysign_t(m, ql) : T(0);
+  return hasl ? (hasr ? in : ql) : (hasr ? qr : T(0));
+}
+
+// The Rusanov speed |un + sign(un) c| of FluxKernel.cpp:225-226 with c >= 0:
+// |un| + c when un != 0 (the sum of two same-signed values rounds
+// symmetrically), and 0 when un is a zero (sycl::sign(0) == 0).  NaN in,
+// NaN out.
+template <typename T>
+__device__ __forceinline__ T rus_speed(T un, T c) {
+  return (un == T(0)) ? T(0) : fabs_t(un) + c;
+}
+
+// a / b with rb = 1/b: exact division (div_c) unless QUICK, where a * rb is
+// used (exact for power-of-two b; the FP32 form uses it always).
+template <bool QUICK, typename T>
+__device__ __forceinline__ T dv(T a, T b, T rb) {
+  if constexpr (QUICK) return a * rb;
+  else return div_c(a, b, rb);
+}
+
+// Ghost side of a closed edge face (FluxKernel.cpp:109-157 with the other
+// side's cell): no depth, no normal velocity, tangential velocity and its
+// normal slope kept, bed raised to the cell's water level plus its depth.
+template <typename T>
+__device__ __forceinline__ Side<T> wall_ghost(const Side<T>& c) {
+  Side<T> g;
+  g.h = T(0); g.un = T(0); g.ut = c.ut; g.z = c.z + c.h * T(2);
+  g.sh = T(0); g.sun = T(0); g.sut = c.sut; g.sz = T(0);
+  return g;
+}
+
+// FluxKernel.cpp:159-289 for a face whose sides are given explicitly (wall
+// faces pass a wall_ghost): branches 1, 3 and 4 as one Rusanov form over the
+// wet sides (a dry side contributes zero, and a side that is wet alone takes
+// +g h on the plus side as branch 4 does), branch 2 selected after.
+template <typename T, bool DEPTHREL, bool QUICK>
+__device__ __forceinline__ Flux<T> face_fast(const Side<T>& L, const Side<T>& R, T hd, T d, T rd,
+                                             float* branch) {
+  const T zm = mad_t(hd, L.sz, L.z);
+  T hm = mad_t(hd, L.sh, L.h);
+  const T unm = mad_t(hd, L.sun, L.un);
+  const T utm = mad_t(hd, L.sut, L.ut);
+  const T zp = mad_t(-hd, R.sz, R.z);
+  T hp = mad_t(-hd, R.sh, R.h);
+  const T unp = mad_t(-hd, R.sun, R.un);
+  const T utp = mad_t(-hd, R.sut, R.ut);
+  bool wm, wp;
+  if (DEPTHREL) {
+    const T dz = zp - zm;
+    wm = hm > fmax_t(dz, T(0));
+    wp = hp > fmax_t(-dz, T(0));
+  } else {
+    const T zf = fmax_t(zm, zp);
+    wm = zm + hm > zf;
+    wp = zp + hp > zf;
+  }
+  hm = fmax_t(hm, T(0));
+  hp = fmax_t(hp, T(0));
+  const T G = T(kG);
+  const T ghm = G * hm, ghp = G * hp;
+  Flux<T> F;
+  if (wm || wp) {
+    const T hme = wm ? hm : T(0), unme = wm ? unm : T(0), utme = wm ? utm : T(0);
+    const T hpe = wp ? hp : T(0), unpe = wp ? unp : T(0), utpe = wp ? utp : T(0);
+    const T cm = wm ? fsqrt_t(ghm) : T(0);
+    const T cp = wp ? fsqrt_t(ghp) : T(0);
+    const T gm = wm ? ghm : T(0);
+    const T gp = wp ? (wm ? -ghp : ghp) : T(0);
+    const T nfm = unme * (T(0.5) * unme) + gm;
+    const T nfp = unpe * (T(0.5) * unpe) + gp;
+    const T a = fmax_t(rus_speed(unpe, cp), rus_speed(unme, cm));
+    const T ha = T(0.5) * a;
+    F.h = T(0.5) * (hpe * unpe + hme * unme) - ha * (hpe - hme);
+    F.n = T(0.5) * (nfp + nfm) - ha * (unpe - unme);
+    F.t = T(0.5) * (utpe * unpe + utme * unme) - ha * (utpe - utme);
+    F.z = (wm && wp) ? (zm - zp) * G : dv<QUICK>(wm ? hp : hm, d, rd);
+    *branch = (wm && wp) ? 1.0f : (wm ? 3.0f : 4.0f);
+  } else {
+    const bool r = zp > zm;
+    F.h = T(0);
+    F.t = T(0);
+    F.n = r ? T(0.5) * ghm : T(-0.5) * ghp;
+    F.z = r ? -hm * T(0.5) * G : hp * T(0.5) * G;
+    *branch = r ? 2.25f : 2.75f;
+  }
   return F;
 }
 

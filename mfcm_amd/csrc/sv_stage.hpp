@@ -1,5 +1,5 @@
-// sv_stage.hpp -- the fused Runge-Kutta stage kernel and the device
-// controller for one catchment on gfx950.
+// sv_stage.hpp -- the fused Runge-Kutta stage kernel for one catchment on
+// gfx950.
 //
 // One stage kernel launch performs, for every cell, what the reference does
 // with ~15 SYCL kernels per stage (SaintVenantSolver::update_dUdt,
@@ -16,10 +16,10 @@
 //
 // Work decomposition: one wavefront owns a strip of 64 columns (60 interior
 // + 2 halo columns each side) and marches down `rows` rows plus a 2-row halo.
-// The three live rows are kept in registers; x-neighbours move between lanes
-// with cross-lane shuffles; each face is evaluated exactly once per wave.
-// Every row load and store is a contiguous 64-element segment of a
-// structure-of-arrays field.
+// The live rows are kept in registers (one row prefetched ahead);
+// x-neighbours move between lanes with cross-lane shuffles; each face and
+// each face difference is evaluated once per wave.  Every row load and store
+// is a contiguous 64-element segment of a structure-of-arrays field.
 #pragma once
 #include "sv_math.hpp"
 
@@ -54,22 +54,28 @@ struct Ctrl {
   int32_t pad;
 };
 
+// Drifting Gaussian storm (extension E4): peak, sigma, centre at t = 0 and
+// drift velocity.
+struct Storm {
+  double peak, sigma, cx0, cy0, vx, vy;
+};
+
 template <typename T>
 struct StageArgs {
-  int64_t nx, ny;
-  int64_t nrb, ntiles;      // row blocks per strip, tiles (= waves)
+  int32_t nx, ny;           // cells per row / rows (nx*ny < 2^31)
+  int32_t nrb, ntiles;      // row blocks per strip, tiles (= waves)
   int32_t rows;             // interior rows per tile
   int32_t stage, nstages;
-  int32_t nacc;             // accumulator rows updated by this stage
   int32_t last;             // last stage: control number + volume
-  int32_t kout;             // write k to k_* instead of accumulating
   int32_t flags;
+  int32_t nanbed;           // the bed has coded-out (NaN) cells
   int32_t manning, discharge, head;
-  int64_t hx0, hx1;
+  int32_t hx0, hx1;
   T dx, dy, rdx, rdy, hdx, hdy;  // spacing, reciprocal, half spacing
-  double dxd, dyd, area;          // FP64 spacing for the storm / volume
+  double dxd, dyd;                // FP64 spacing for the storm
   double ox, oy;
   double c_s;               // c coefficient of this stage
+  double rstep;             // 1 / outer step duration
   double acoef[kMaxStages]; // a_{i, s+1} for the rows this stage feeds
   // inputs
   const T* in_h; const T* in_u; const T* in_v;   // stage input; null = U0
@@ -79,7 +85,7 @@ struct StageArgs {
   const T* q0f; const T* q1f; const T* hb0f; const T* hb1f;
   T n_sh, n_dp, d_sh, d_dp;
   T h_dry, u_max;           // E8 dry depth, E9 velocity cap
-  double storm[6];          // peak, sigma, cx0, cy0, vx, vy
+  Storm storm;
   // accumulators: dst[j]; null = the new-state buffer U[cur^1]
   T* acc_h[kMaxStages]; T* acc_u[kMaxStages]; T* acc_v[kMaxStages];
   T* k_h; T* k_u; T* k_v;
@@ -90,7 +96,6 @@ struct StageArgs {
 };
 
 template <typename T> struct Row { T h, u, v, z; };
-template <typename T> struct Slopes { T hx, ux, vx, zx, hy, uy, vy, zy; };
 template <typename T> struct FaceUV { T h, u, v, z; };
 
 __device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, kWave); }
@@ -98,71 +103,120 @@ __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWav
 __device__ __forceinline__ double shfl_dn1(double v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float shfl_dn1(float v) { return __shfl_down(v, 1, kWave); }
 
+template <typename T>
+__device__ __forceinline__ Row<T> shfl_up_row(const Row<T>& r) {
+  return {shfl_up1(r.h), shfl_up1(r.u), shfl_up1(r.v), shfl_up1(r.z)};
+}
+template <typename T>
+__device__ __forceinline__ Row<T> shfl_dn_row(const Row<T>& r) {
+  return {shfl_dn1(r.h), shfl_dn1(r.u), shfl_dn1(r.v), shfl_dn1(r.z)};
+}
+template <typename T>
+__device__ __forceinline__ FaceUV<T> shfl_dn_face(const FaceUV<T>& r) {
+  return {shfl_dn1(r.h), shfl_dn1(r.u), shfl_dn1(r.v), shfl_dn1(r.z)};
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that runs
-// of consecutive tiles (vertically adjacent strips sharing halo rows) land on
+// of consecutive tiles (vertically adjacent tiles sharing halo rows) land on
 // one XCD and its L2.
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
-  const int64_t nb8 = (nb / 8) * 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int nb8 = (nb / 8) * 8;
   if (b >= nb8) return b;
   return (b % 8) * (nb8 / 8) + b / 8;
 }
 
+// One separable factor of the E4 storm, exp(-(a - c)^2 / (2 sigma^2)) for
+// coordinate a and centre c (the oracle's storm_axis).
 template <typename T>
-__device__ __forceinline__ double storm_rate(const double* s, double x, double y, double t) {
-  const double cx = s[2] + s[4] * t, cy = s[3] + s[5] * t;
-  const double ddx = x - cx, ddy = y - cy;
-  return s[0] * exp(-(ddx * ddx + ddy * ddy) / (2.0 * s[1] * s[1]));
-}
-template <>
-__device__ __forceinline__ double storm_rate<float>(const double* s, double x, double y, double t) {
-  const float cx = float(s[2] + s[4] * t), cy = float(s[3] + s[5] * t);
-  const float ddx = float(x) - cx, ddy = float(y) - cy;
-  const float sg = float(s[1]);
-  return double(float(s[0]) * __expf(-(ddx * ddx + ddy * ddy) / (2.0f * sg * sg)));
+__device__ __forceinline__ T storm_axis(const Storm& s, double a, double c) {
+  if constexpr (sizeof(T) == 8) {
+    const double d = a - c;
+    return exp_neg(-(d * d) / (2.0 * s.sigma * s.sigma));
+  } else {
+    const float d = float(a - c), sg = float(s.sigma);
+    return __expf(-(d * d) / (2.0f * sg * sg));
+  }
 }
 
-template <typename T>
+// Division by a per-cell value: correctly rounded in FP64 (the reference's
+// a / b), reciprocal-based in the FP32 form.
+__device__ __forceinline__ double vdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float vdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// NACC: accumulator rows this stage writes (0 with KOUT: write k instead).
+// QUICK: divide by dx, dy with one multiplication by the reciprocal (exact
+// for power-of-two spacings; the FP32 form always does).
+template <typename T, int NACC, bool KOUT, bool QUICK>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArgs<T> A) {
   constexpr bool kRel = sizeof(T) == 4;
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t tile = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  const int tile = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
   if (tile >= A.ntiles) return;
   Ctrl* __restrict__ C = A.ctrl;
   if (C->done) return;
 
-  const int64_t nx = A.nx, ny = A.ny;
-  const int64_t strip = tile / A.nrb, rb = tile % A.nrb;
-  const int64_t x = strip * kStrip - kHalo + lane;
-  const int64_t y0 = rb * A.rows;
-  const int64_t y1 = (y0 + A.rows < ny) ? y0 + A.rows : ny;
+  const int nx = A.nx, ny = A.ny;
+  const int strip = tile / A.nrb, rb = tile - strip * A.nrb;
+  const int x = strip * kStrip - kHalo + lane;
+  const int y0 = rb * A.rows;
+  const int y1 = (y0 + A.rows < ny) ? y0 + A.rows : ny;
   const bool xin = x >= 0 && x < nx;
   const bool own = lane >= kHalo && lane < kWave - kHalo && xin;
   const bool hasW = x > 0, hasE = x < nx - 1;
+  const bool e7 = (A.flags & 0x40) != 0;
+  const bool nanbed = A.nanbed != 0;
 
   const double dt = C->dt;
   const double tsub = C->t_local + A.c_s * dt;
   const int cur = C->cur;
-  const T* __restrict__ xh = A.in_h ? A.in_h : A.Uh[cur];
-  const T* __restrict__ xu = A.in_u ? A.in_u : A.Uu[cur];
-  const T* __restrict__ xv = A.in_v ? A.in_v : A.Uv[cur];
+  // select, not index: a run-time index into the argument block would copy
+  // it to private memory
+  T* const Uh0 = cur ? A.Uh[1] : A.Uh[0], * const Uh1 = cur ? A.Uh[0] : A.Uh[1];
+  T* const Uu0 = cur ? A.Uu[1] : A.Uu[0], * const Uu1 = cur ? A.Uu[0] : A.Uu[1];
+  T* const Uv0 = cur ? A.Uv[1] : A.Uv[0], * const Uv1 = cur ? A.Uv[0] : A.Uv[1];
+  const T* __restrict__ xh = A.in_h ? A.in_h : Uh0;
+  const T* __restrict__ xu = A.in_u ? A.in_u : Uu0;
+  const T* __restrict__ xv = A.in_v ? A.in_v : Uv0;
   const T* __restrict__ zb = A.zb;
   const T dx = A.dx, dy = A.dy, rdx = A.rdx, rdy = A.rdy, hdx = A.hdx, hdy = A.hdy;
   const T G = T(kG);
 
   // RK coefficients of this stage: (T)(a * dt) as State::operator*(ValueType).
-  T sc[kMaxStages];
+  T sc[NACC > 0 ? NACC : 1];
+  T* dh_[NACC > 0 ? NACC : 1];
+  T* du_[NACC > 0 ? NACC : 1];
+  T* dv_[NACC > 0 ? NACC : 1];
 #pragma unroll
-  for (int j = 0; j < kMaxStages; ++j) sc[j] = T(A.acoef[j] * dt);
+  for (int j = 0; j < NACC; ++j) {
+    sc[j] = T(A.acoef[j] * dt);
+    dh_[j] = A.acc_h[j] ? A.acc_h[j] : Uh1;
+    du_[j] = A.acc_u[j] ? A.acc_u[j] : Uu1;
+    dv_[j] = A.acc_v[j] ? A.acc_v[j] : Uv1;
+  }
 
   // Uniform discharge / head terms (global selection, scalar endpoints).
   const double step_len = C->step_dur;
-  double q0s = C->q0, q1s = C->q1, hb0s = C->hb0, hb1s = C->hb1;
+  const double q0s = C->q0, q1s = C->q1, hb0s = C->hb0, hb1s = C->hb1;
   const double gstep = C->gstep;
+  const bool storm = (A.flags & 0x08) != 0;
+  // E4 storm factors: this lane's column at the step's two endpoint times,
+  // and (in lane k) row y0 + k, broadcast to all lanes row by row.
+  T ex0 = T(0), ex1 = T(0), eyA0 = T(0), eyA1 = T(0);
+  if (storm) {
+    const Storm& st = A.storm;
+    const double t1 = gstep + step_len;
+    const double xc = A.ox + (double(x) + 0.5) * A.dxd;
+    ex0 = storm_axis<T>(st, xc, st.cx0 + st.vx * gstep);
+    ex1 = storm_axis<T>(st, xc, st.cx0 + st.vx * t1);
+    const double yc = A.oy + (double(y0 + lane) + 0.5) * A.dyd;
+    eyA0 = storm_axis<T>(st, yc, st.cy0 + st.vy * gstep);
+    eyA1 = storm_axis<T>(st, yc, st.cy0 + st.vy * t1);
+  }
 
-  auto load = [&](int64_t r) {
+  auto load = [&](int r) {
     Row<T> R;
     if (xin && r >= 0 && r < ny) {
-      const int64_t i = r * nx + x;
+      const int i = r * nx + x;
       R.h = xh[i]; R.u = xu[i]; R.v = xv[i]; R.z = zb[i];
     } else {
       R.h = T(0); R.u = T(0); R.v = T(0); R.z = T(0);
@@ -170,119 +224,189 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArg
     return R;
   };
 
-  Row<T> R0 = load(y0 - 2), R1 = load(y0 - 1), R2;
-  Slopes<T> S0 = {}, S1 = {};
-  FaceUV<T> Fs0 = {}, Fs1 = {}, Xw0 = {}, Xw1 = {};
-  double vol = 0.0, rate = 0.0;
-  double cnmax = 0.0;
+  Row<T> R0 = load(y0 - 2), R1 = load(y0 - 1), R2 = load(y0);
+  // Per-cell inputs of the cell pass (Manning n, accumulator bases of a
+  // later stage), loaded one row before the row that uses them.
+  constexpr int NB = NACC > 0 ? NACC : 1;
+  struct Extra {
+    T n;
+    T bh[NB], bu[NB], bv[NB];
+  };
+  const bool base_ld = A.stage > 0;
+  auto load_extra = [&](int r) {
+    Extra e;
+    e.n = A.n_sh;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) { e.bh[j] = T(0); e.bu[j] = T(0); e.bv[j] = T(0); }
+    if (own && r >= y0 && r < y1) {
+      const int i = r * nx + x;
+      if (A.nfield) e.n = A.nfield[i];
+      if (base_ld) {
+#pragma unroll
+        for (int j = 0; j < NACC; ++j) { e.bh[j] = dh_[j][i]; e.bu[j] = du_[j][i]; e.bv[j] = dv_[j][i]; }
+      }
+    }
+    return e;
+  };
+  Extra X0 = load_extra(y0 - 1);
+  // south quotients (U_c - U_{c-1})/dy of the current row, carried down
+  // (first row: rows y0-2 and y0-1; unused when y0-1 has no south neighbour)
+  T qsh = dv<QUICK>(R1.h - R0.h, dy, rdy), qsu = dv<QUICK>(R1.u - R0.u, dy, rdy);
+  T qsv = dv<QUICK>(R1.v - R0.v, dy, rdy), qsz = dv<QUICK>(R1.z - R0.z, dy, rdy);
+  // slopes of row c-1 that the cell pass and the y-face still need
+  T p_hx = T(0), p_zx = T(0), p_hy = T(0), p_uy = T(0), p_vy = T(0), p_zy = T(0);
+  FaceUV<T> Fs0 = {}, Xw0 = {};
+  double vol = 0.0, rate = 0.0, cnmax = 0.0;
 
-  for (int64_t c = y0 - 1; c <= y1; ++c) {
-    R2 = load(c + 1);
+#pragma unroll 3
+  for (int c = y0 - 1; c <= y1; ++c) {
+    const Row<T> Rn = load(c + 2);  // prefetch
+    const Extra X1 = load_extra(c);
     const bool rowin = c >= 0 && c < ny;
+    const Row<T> W = shfl_up_row(R1), E = shfl_dn_row(R1);
     // ---- slopes of row c (State.cpp:187-192, SpatialDerivative.hpp:104-122)
-    Row<T> W, E;
-    W.h = shfl_up1(R1.h); W.u = shfl_up1(R1.u); W.v = shfl_up1(R1.v); W.z = shfl_up1(R1.z);
-    E.h = shfl_dn1(R1.h); E.u = shfl_dn1(R1.u); E.v = shfl_dn1(R1.v); E.z = shfl_dn1(R1.z);
+    T shx = T(0), sux = T(0), svx = T(0), szx = T(0);
+    T shy = T(0), suy = T(0), svy = T(0), szy = T(0);
+    T qnh = T(0), qnu = T(0), qnv = T(0), qnz = T(0);
     if (rowin) {
-      const T wh = hasW ? W.h : R1.h, wu = hasW ? W.u : R1.u, wv = hasW ? W.v : R1.v, wz = hasW ? W.z : R1.z;
-      const T eh = hasE ? E.h : R1.h, eu = hasE ? E.u : R1.u, ev = hasE ? E.v : R1.v, ez = hasE ? E.z : R1.z;
-      S1.hx = minmod3(wh, hasW, R1.h, hasE, eh, dx, rdx);
-      S1.ux = minmod3(wu, hasW, R1.u, hasE, eu, dx, rdx);
-      S1.vx = minmod3(wv, hasW, R1.v, hasE, ev, dx, rdx);
-      S1.zx = minmod3(wz, hasW, R1.z, hasE, ez, dx, rdx);
-      const bool hasN = c + 1 < ny, hasS = c - 1 >= 0;
-      const Row<T> N = hasN ? R2 : R1;
-      const Row<T> S = hasS ? R0 : R1;
-      if (A.flags & 0x40) {  // E7: south as the left neighbour
-        S1.hy = minmod3(S.h, hasS, R1.h, hasN, N.h, dy, rdy);
-        S1.uy = minmod3(S.u, hasS, R1.u, hasN, N.u, dy, rdy);
-        S1.vy = minmod3(S.v, hasS, R1.v, hasN, N.v, dy, rdy);
-        S1.zy = minmod3(S.z, hasS, R1.z, hasN, N.z, dy, rdy);
-      } else {               // reference: north as the left neighbour
-        S1.hy = minmod3(N.h, hasN, R1.h, hasS, S.h, dy, rdy);
-        S1.uy = minmod3(N.u, hasN, R1.u, hasS, S.u, dy, rdy);
-        S1.vy = minmod3(N.v, hasN, R1.v, hasS, S.v, dy, rdy);
-        S1.zy = minmod3(N.z, hasN, R1.z, hasS, S.z, dy, rdy);
+      const bool hasN = c + 1 < ny, hasS = c > 0;
+      if (!nanbed) {
+        const T qwh = dv<QUICK>(R1.h - W.h, dx, rdx), qwu = dv<QUICK>(R1.u - W.u, dx, rdx);
+        const T qwv = dv<QUICK>(R1.v - W.v, dx, rdx), qwz = dv<QUICK>(R1.z - W.z, dx, rdx);
+        const T qeh = shfl_dn1(qwh), qeu = shfl_dn1(qwu), qev = shfl_dn1(qwv), qez = shfl_dn1(qwz);
+        shx = minmod_q(qwh, qeh, dv<QUICK>(E.h - W.h, dx, rdx), hasW, hasE);
+        sux = minmod_q(qwu, qeu, dv<QUICK>(E.u - W.u, dx, rdx), hasW, hasE);
+        svx = minmod_q(qwv, qev, dv<QUICK>(E.v - W.v, dx, rdx), hasW, hasE);
+        szx = minmod_q(qwz, qez, dv<QUICK>(E.z - W.z, dx, rdx), hasW, hasE);
+        qnh = dv<QUICK>(R2.h - R1.h, dy, rdy); qnu = dv<QUICK>(R2.u - R1.u, dy, rdy);
+        qnv = dv<QUICK>(R2.v - R1.v, dy, rdy); qnz = dv<QUICK>(R2.z - R1.z, dy, rdy);
+        // south -> north slope; the reference's north -> south slope is its
+        // exact negation (Minmod3 commutes with negation)
+        const T yh = minmod_q(qsh, qnh, dv<QUICK>(R2.h - R0.h, dy, rdy), hasS, hasN);
+        const T yu = minmod_q(qsu, qnu, dv<QUICK>(R2.u - R0.u, dy, rdy), hasS, hasN);
+        const T yv = minmod_q(qsv, qnv, dv<QUICK>(R2.v - R0.v, dy, rdy), hasS, hasN);
+        const T yz = minmod_q(qsz, qnz, dv<QUICK>(R2.z - R0.z, dy, rdy), hasS, hasN);
+        shy = e7 ? yh : -yh; suy = e7 ? yu : -yu; svy = e7 ? yv : -yv; szy = e7 ? yz : -yz;
+      } else {
+        // general Minmod3 with the reference's NaN rules
+        const Row<T> Wc = hasW ? W : R1, Ec = hasE ? E : R1;
+        shx = minmod3(Wc.h, hasW, R1.h, hasE, Ec.h, dx, rdx);
+        sux = minmod3(Wc.u, hasW, R1.u, hasE, Ec.u, dx, rdx);
+        svx = minmod3(Wc.v, hasW, R1.v, hasE, Ec.v, dx, rdx);
+        szx = minmod3(Wc.z, hasW, R1.z, hasE, Ec.z, dx, rdx);
+        const Row<T> N = hasN ? R2 : R1, S = hasS ? R0 : R1;
+        if (e7) {
+          shy = minmod3(S.h, hasS, R1.h, hasN, N.h, dy, rdy);
+          suy = minmod3(S.u, hasS, R1.u, hasN, N.u, dy, rdy);
+          svy = minmod3(S.v, hasS, R1.v, hasN, N.v, dy, rdy);
+          szy = minmod3(S.z, hasS, R1.z, hasN, N.z, dy, rdy);
+        } else {
+          shy = minmod3(N.h, hasN, R1.h, hasS, S.h, dy, rdy);
+          suy = minmod3(N.u, hasN, R1.u, hasS, S.u, dy, rdy);
+          svy = minmod3(N.v, hasN, R1.v, hasS, S.v, dy, rdy);
+          szy = minmod3(N.z, hasN, R1.z, hasS, S.z, dy, rdy);
+        }
       }
     }
     // ---- west x-face of row c (face id c*(nx+1) + x)
-    const T Whx = shfl_up1(S1.hx), Wux = shfl_up1(S1.ux), Wvx = shfl_up1(S1.vx), Wzx = shfl_up1(S1.zx);
+    const T Whx = shfl_up1(shx), Wux = shfl_up1(sux), Wvx = shfl_up1(svx), Wzx = shfl_up1(szx);
+    FaceUV<T> Xw1 = {};
     if (c >= y0 && c < y1) {
-      Side<T> L, R;
-      R = {R1.h, R1.u, R1.v, R1.z, S1.hx, S1.ux, S1.vx, S1.zx};
-      int edge = 0;
-      if (x == 0) {
-        if (A.flags & 0x02) {  // E2: zero-gradient ghost that only drains
-          L = {R1.h, fmin_t(R1.u, T(0)), R1.v, R1.z, T(0), T(0), T(0), T(0)};
-        } else {
-          L = R;
-          edge = -1;
-        }
-      } else {
-        L = {W.h, W.u, W.v, W.z, Whx, Wux, Wvx, Wzx};
-        if (x == nx) { R = L; edge = 1; }
-      }
+      Side<T> R = {R1.h, R1.u, R1.v, R1.z, shx, sux, svx, szx};
+      Side<T> L = {W.h, W.u, W.v, W.z, Whx, Wux, Wvx, Wzx};
       float br;
-      const Flux<T> F = face_flux<T, kRel>(L, R, edge, hdx, dx, rdx, &br);
+      Flux<T> F;
+      if (x == 0 && (A.flags & 0x02)) {  // E2: zero-gradient ghost that only drains
+        L = {R1.h, fmin_t(R1.u, T(0)), R1.v, R1.z, T(0), T(0), T(0), T(0)};
+        F = face_flux<T, kRel>(L, R, 0, hdx, dx, rdx, &br);
+      } else if (!nanbed) {
+        if (x == 0) L = wall_ghost(R);
+        else if (x == nx) R = wall_ghost(L);
+        F = face_fast<T, kRel, QUICK>(L, R, hdx, dx, rdx, &br);
+      } else {
+        int edge = 0;
+        if (x == 0) { L = R; edge = -1; }
+        else if (x == nx) { R = L; edge = 1; }
+        F = face_flux<T, kRel>(L, R, edge, hdx, dx, rdx, &br);
+      }
       Xw1 = {F.h, F.n, F.t, F.z};
       if (A.br_x && x >= 0 && x <= nx && lane >= kHalo && lane <= kWave - kHalo)
         A.br_x[c * (nx + 1) + x] = br;
     }
     // ---- south y-face of row c (face id (nx+1)ny + c*nx + x)
+    FaceUV<T> Fs1 = {};
     if (c >= y0) {
       Side<T> L, R;
-      int edge = 0;
-      if (c == ny) {          // north edge of the top row
-        L = {R0.h, R0.v, R0.u, R0.z, S0.hy, S0.vy, S0.uy, S0.zy};
-        R = L;
-        edge = 1;
-      } else {
-        R = {R1.h, R1.v, R1.u, R1.z, S1.hy, S1.vy, S1.uy, S1.zy};
-        if (c == 0) { L = R; edge = -1; }
-        else L = {R0.h, R0.v, R0.u, R0.z, S0.hy, S0.vy, S0.uy, S0.zy};
-      }
       float br;
-      const Flux<T> F = face_flux<T, kRel>(L, R, edge, hdy, dy, rdy, &br);
+      Flux<T> F;
+      if (c == ny) {  // north edge of the top row
+        L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
+        if (!nanbed) F = face_fast<T, kRel, QUICK>(L, wall_ghost(L), hdy, dy, rdy, &br);
+        else F = face_flux<T, kRel>(L, L, 1, hdy, dy, rdy, &br);
+      } else {
+        R = {R1.h, R1.v, R1.u, R1.z, shy, svy, suy, szy};
+        if (c == 0) {
+          if (!nanbed) F = face_fast<T, kRel, QUICK>(wall_ghost(R), R, hdy, dy, rdy, &br);
+          else F = face_flux<T, kRel>(R, R, -1, hdy, dy, rdy, &br);
+        } else {
+          L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
+          if (!nanbed) F = face_fast<T, kRel, QUICK>(L, R, hdy, dy, rdy, &br);
+          else F = face_flux<T, kRel>(L, R, 0, hdy, dy, rdy, &br);
+        }
+      }
       Fs1 = {F.h, F.t, F.n, F.z};
-      if (A.br_y && own) A.br_y[(nx + 1) * ny + c * nx + x - (nx + 1) * ny] = br;
+      if (A.br_y && own) A.br_y[c * nx + x] = br;
     }
     // ---- cell pass + sources + RK for row r = c - 1
-    const FaceUV<T> Xe0 = {shfl_dn1(Xw0.h), shfl_dn1(Xw0.u), shfl_dn1(Xw0.v), shfl_dn1(Xw0.z)};
-    const int64_t r = c - 1;
+    const FaceUV<T> Xe0 = shfl_dn_face(Xw0);
+    const int r = c - 1;
+    T ey0 = T(0), ey1 = T(0);
+    if (storm) {  // uniform: every lane takes part in the broadcast
+      const int k = r - y0;
+      if (k >= 0 && k < kWave) {
+        ey0 = __shfl(eyA0, k, kWave);
+        ey1 = __shfl(eyA1, k, kWave);
+      } else {
+        const Storm& st = A.storm;
+        const double yc = A.oy + (double(r) + 0.5) * A.dyd;
+        ey0 = storm_axis<T>(st, yc, st.cy0 + st.vy * gstep);
+        ey1 = storm_axis<T>(st, yc, st.cy0 + st.vy * (gstep + step_len));
+      }
+    }
     if (r >= y0 && r < y1 && own) {
-      const int64_t i = r * nx + x;
+      const int i = r * nx + x;
       const T h = R0.h, u = R0.u, v = R0.v;
       // TemporalDerivativeKernel.cpp:57-90
-      T dhdt = div_c(Xw0.h - Xe0.h, dx, rdx) + div_c(Fs0.h - Fs1.h, dy, rdy);
-      T dudt = div_c(Xw0.u - Xe0.u, dx, rdx) + div_c(Fs0.u - Fs1.u, dy, rdy);
-      T dvdt = div_c(Xw0.v - Xe0.v, dx, rdx) + div_c(Fs0.v - Fs1.v, dy, rdy);
-      dudt += S0.hx * T(-kG);
-      dvdt += S0.hy * T(-kG);
-      T dzdx = S0.zx;
-      const T hx_ = div_c(h, dx, rdx);
+      T dhdt = dv<QUICK>(Xw0.h - Xe0.h, dx, rdx) + dv<QUICK>(Fs0.h - Fs1.h, dy, rdy);
+      T dudt = dv<QUICK>(Xw0.u - Xe0.u, dx, rdx) + dv<QUICK>(Fs0.u - Fs1.u, dy, rdy);
+      T dvdt = dv<QUICK>(Xw0.v - Xe0.v, dx, rdx) + dv<QUICK>(Fs0.v - Fs1.v, dy, rdy);
+      dudt += p_hx * T(-kG);
+      dvdt += p_hy * T(-kG);
+      T dzdx = p_zx;
+      const T hx_ = dv<QUICK>(h, dx, rdx);
       if (fabs_t(dzdx) > hx_) dzdx = sy_sign(dzdx) * hx_;
-      T dzdy = S0.zy;
-      const T hy_ = div_c(h, dy, rdy);
+      T dzdy = p_zy;
+      const T hy_ = dv<QUICK>(h, dy, rdy);
       if (fabs_t(dzdy) > hy_) dzdy = sy_sign(dzdy) * hy_;
       const T dudt_bed = T(-kG) * dzdx;
       const T dvdt_bed = T(-kG) * dzdy;
-      dudt += div_c(Xw0.z - Xe0.z, dx, rdx);
-      dvdt += div_c(Fs0.z - Fs1.z, dy, rdy);
+      dudt += dv<QUICK>(Xw0.z - Xe0.z, dx, rdx);
+      dvdt += dv<QUICK>(Fs0.z - Fs1.z, dy, rdy);
       dudt += dudt_bed;
       dvdt += dvdt_bed;
       // ManningRoughnessSourceTerm.cpp:55-90
       if (A.manning) {
         T nsh = A.n_sh, ndp = A.n_dp;
-        if (A.nfield) { nsh = A.nfield[i]; ndp = nsh; }
-        T n = nsh;
+        if (A.nfield) { nsh = X0.n; ndp = nsh; }
+        T n = nsh;  // mix(a, a, t) == a
         if (nsh != ndp) {
-          T t = (h - A.d_sh) / (A.d_dp - A.d_sh);
+          T t = vdiv(h - A.d_sh, A.d_dp - A.d_sh);
           t = fmin_t(fmax_t(t, T(0)), T(1));
           n = nsh + (ndp - nsh) * (t * t * (T(3) - T(2) * t));
         }
         if (h > T(1e-6)) {
-          const T inv_h = h / (h * h + T(1e-3));
-          const T Sf = n * n * sqrt_t(u * u + v * v) * pow43(inv_h);
+          const T inv_h = vdiv(h, h * h + T(1e-3));
+          const T Sf = n * n * fsqrt_t(u * u + v * v) * pow43(inv_h);
           T dum = T(-kG) * Sf * u;
           T dvm = T(-kG) * Sf * v;
           const double ts = dt;
@@ -293,23 +417,28 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArg
         }
       }
       // DischargeBoundarySourceTerm.cpp:43-67 (E1 flips the guard)
-      if (A.discharge || (A.flags & 0x08)) {
-        double q0, q1;
-        if (A.discharge == 2) { q0 = A.q0f[i]; q1 = A.q1f[i]; }
-        else if (A.discharge == 1) { q0 = q0s; q1 = q1s; }
-        else { q0 = 0.0; q1 = 0.0; }
-        if (A.flags & 0x08) {
-          const double cxp = A.ox + (double(x) + 0.5) * A.dxd;
-          const double cyp = A.oy + (double(r) + 0.5) * A.dyd;
-          q0 += storm_rate<T>(A.storm, cxp, cyp, gstep);
-          q1 += storm_rate<T>(A.storm, cxp, cyp, gstep + step_len);
+      if (A.discharge || storm) {
+        T tq0, tq1;
+        if (A.discharge == 2) { tq0 = A.q0f[i]; tq1 = A.q1f[i]; }
+        else if (A.discharge == 1) { tq0 = T(q0s); tq1 = T(q1s); }
+        else { tq0 = T(0); tq1 = T(0); }
+        if (storm) {
+          tq0 += T(A.storm.peak) * ex0 * ey0;
+          tq1 += T(A.storm.peak) * ex1 * ey1;
         }
-        const T tq0 = T(q0), tq1 = T(q1);
         if (tq0 != T(0) || tq1 != T(0)) {
-          const T dqdt = T((tq1 - tq0) / step_len);
-          const T qnow = T(tq0 + tsub * dqdt);
-          const T qnext = T(qnow + dt * dqdt);
-          T dh = T(0.5) * (qnow + qnext);
+          T dh;
+          if constexpr (sizeof(T) == 8) {
+            const double dqdt = div_c(tq1 - tq0, step_len, A.rstep);
+            const double qnow = tq0 + tsub * dqdt;
+            const double qnext = qnow + dt * dqdt;
+            dh = 0.5 * (qnow + qnext);
+          } else {
+            const float dqdt = (tq1 - tq0) * float(A.rstep);
+            const float qnow = __builtin_fmaf(float(tsub), dqdt, tq0);
+            const float qnext = __builtin_fmaf(float(dt), dqdt, qnow);
+            dh = 0.5f * (qnow + qnext);
+          }
           const bool dry = (A.flags & 0x01) ? (h + dh * dt <= 0.0) : (h - dh * dt <= 0.0);
           if (dry) dh = T(-h / dt);
           dhdt += dh;
@@ -331,44 +460,42 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArg
         }
       }
       rate += double(dhdt);
-      if (A.kout) {
+      if (KOUT) {
         A.k_h[i] = dhdt; A.k_u[i] = dudt; A.k_v[i] = dvdt;
       } else {
         // RungeKutta.hpp:377-388: y_i = y_i + k_s * (a dt)
         T oh = h, ou = u, ov = v;
 #pragma unroll
-        for (int j = 0; j < kMaxStages; ++j) {
-          if (j < A.nacc) {
-            T* dh_ = A.acc_h[j] ? A.acc_h[j] : A.Uh[cur ^ 1];
-            T* du_ = A.acc_u[j] ? A.acc_u[j] : A.Uu[cur ^ 1];
-            T* dv_ = A.acc_v[j] ? A.acc_v[j] : A.Uv[cur ^ 1];
-            T bh = h, bu = u, bv = v;
-            if (A.stage > 0) { bh = dh_[i]; bu = du_[i]; bv = dv_[i]; }
-            oh = bh + dhdt * sc[j];
-            ou = bu + dudt * sc[j];
-            ov = bv + dvdt * sc[j];
-            if (j == 0) {  // row s+1 is complete after this stage
-              if ((A.flags & 0x80) && oh <= A.h_dry) { ou = T(0); ov = T(0); }  // E8
-              if (A.flags & 0x100) {                                           // E9
-                ou = fmin_t(fmax_t(ou, -A.u_max), A.u_max);
-                ov = fmin_t(fmax_t(ov, -A.u_max), A.u_max);
-              }
+        for (int j = 0; j < NACC; ++j) {
+          T bh = h, bu = u, bv = v;
+          if (base_ld) { bh = X0.bh[j]; bu = X0.bu[j]; bv = X0.bv[j]; }
+          T nh = bh + dhdt * sc[j];
+          T nu = bu + dudt * sc[j];
+          T nv = bv + dvdt * sc[j];
+          if (j == 0) {  // row s+1 is complete after this stage
+            if ((A.flags & 0x80) && nh <= A.h_dry) { nu = T(0); nv = T(0); }  // E8
+            if (A.flags & 0x100) {                                           // E9
+              nu = fmin_t(fmax_t(nu, -A.u_max), A.u_max);
+              nv = fmin_t(fmax_t(nv, -A.u_max), A.u_max);
             }
-            dh_[i] = oh; du_[i] = ou; dv_[i] = ov;
+            oh = nh; ou = nu; ov = nv;
           }
+          dh_[j][i] = nh; du_[j][i] = nu; dv_[j][i] = nv;
         }
         if (A.last) {
           // State.cpp:211-222 on the new state
           const T hh = fmax_t(oh, T(0));
-          const T cc = sqrt_t(G * hh);
-          const double cn = dt * double(div_c(fabs_t(ou) + cc, dx, rdx) + div_c(fabs_t(ov) + cc, dy, rdy));
+          const T cc = fsqrt_t(G * hh);
+          const double cn = dt * double(dv<QUICK>(fabs_t(ou) + cc, dx, rdx) + dv<QUICK>(fabs_t(ov) + cc, dy, rdy));
           if (cn > cnmax) cnmax = cn;
           vol += double(oh);
         }
       }
     }
-    R0 = R1; R1 = R2;
-    S0 = S1;
+    R0 = R1; R1 = R2; R2 = Rn;
+    X0 = X1;
+    qsh = qnh; qsu = qnu; qsv = qnv; qsz = qnz;
+    p_hx = shx; p_zx = szx; p_hy = shy; p_uy = suy; p_vy = svy; p_zy = szy;
     Fs0 = Fs1;
     Xw0 = Xw1;
   }
@@ -390,100 +517,6 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArg
       atomicMax(&C->cn_bits, (unsigned long long)__double_as_longlong(cnmax));
     }
   }
-}
-
-struct CtrlArgs {
-  Ctrl* ctrl;
-  const double* part_vol;   // [ntiles]
-  const double* part_rate;  // [nstages][ntiles]
-  int64_t ntiles;
-  int32_t nstages;
-  int32_t mode;             // 0 = device controller, 1 = host-driven (record only)
-  double b[kMaxStages];
-  double area;
-  const double* forcing;    // [nforcing][4] or null
-  int64_t nforcing;
-};
-
-// Fixed-order block sum: thread t sums tiles t, t+256, ... then a tree.
-__device__ __forceinline__ double block_sum(const double* p, int64_t n, double* sh) {
-  double s = 0.0;
-  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) s += p[k];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
-  }
-  const double r = sh[0];
-  __syncthreads();
-  return r;
-}
-
-// Controller (RungeKutta.cpp:154-167) + accept (RungeKutta.hpp:428-431) +
-// landing rules and outer-step transitions (TemporalScheme.hpp:112-149).
-__global__ __launch_bounds__(256) void k_ctrl(const CtrlArgs A) {
-  __shared__ double sh[256];
-  Ctrl* C = A.ctrl;
-  if (C->done) return;
-  const double vol = block_sum(A.part_vol, A.ntiles, sh) * A.area;
-  double rate = 0.0;
-  for (int s = 0; s < A.nstages; ++s)
-    rate += A.b[s] * block_sum(A.part_rate + (int64_t)s * A.ntiles, A.ntiles, sh);
-  if (threadIdx.x != 0) return;
-  const double cn_raw = __longlong_as_double((long long)C->cn_bits);
-  C->cn_bits = 0ull;
-  C->cn = cn_raw;
-  if (A.mode == 1) { C->volume = vol; return; }
-  const double cn = (C->cn_target == 1.0) ? cn_raw : cn_raw / C->cn_target;
-  const double dt = C->dt;
-  bool repeat;
-  double ndt;
-  if (cn > 1.0) {
-    repeat = true;
-    ndt = (cn > 5.0) ? dt / 5.0 : dt / (cn * 1.1);
-  } else if (cn > 0.9) {
-    repeat = false; ndt = dt;
-  } else {
-    repeat = false; ndt = dt / cn;
-  }
-  C->repeat = repeat ? 1 : 0;
-  if (repeat) {
-    C->n_rej++;
-  } else {
-    C->n_acc++;
-    C->t_local += dt;
-    C->cur ^= 1;
-    C->volume = vol;
-    C->srcvol += dt * rate * A.area;
-  }
-  C->dt = ndt;
-  const double sd = C->step_dur, tl = C->t_local;
-  if (tl >= sd) {
-    C->step++;
-    C->t_local = 0.0;
-    // TemporalScheme.hpp:254: time_now = start_time + i * step_duration
-    C->gstep = C->start_time + double(C->step) * sd;
-    if (C->step >= C->nsteps) {
-      C->done = 1;
-    } else if (A.forcing && C->step < A.nforcing) {
-      const double* f = A.forcing + 4 * C->step;
-      C->q0 = f[0]; C->q1 = f[1]; C->hb0 = f[2]; C->hb1 = f[3];
-    }
-  } else if (tl + ndt > sd) {
-    C->dt = sd - tl;
-  } else if (tl + 2.0 * ndt > sd) {
-    C->dt = (sd - tl) * 0.4;
-  }
-}
-
-// Host-driven mode: set the attempt's t_local / dt (and the buffer flip of an
-// accept) without a memcpy node, so the same launches can be graph-captured.
-__global__ void k_set_ctrl(Ctrl* C, double t_local, double dt, int flip, int set_time) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (flip) C->cur ^= 1;
-  if (set_time) { C->t_local = t_local; C->dt = dt; }
-  C->cn_bits = 0ull;
 }
 
 }  // namespace mfsv

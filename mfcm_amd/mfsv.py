@@ -99,6 +99,8 @@ SIGNATURES = {
     "mfsv_set_params": (_I, [_P, C.POINTER(Params)]),
     "mfsv_set_scheme": (_I, [_P, _I, _P]),
     "mfsv_set_time": (_I, [_P, _D, _D, _D, _D]),
+    "mfsv_set_tile_rows": (_I, [_P, _I]),
+    "mfsv_get_tile_rows": (_I, [_P, C.POINTER(C.c_int)]),
     "mfsv_start_new_step": (_I, [_P, _D, _D, _D, _D, _D]),
     "mfsv_set_forcing_table": (_I, [_P, _I64, _P]),
     "mfsv_upload": (_I, [_P, _I, _P, _I64]),
@@ -270,6 +272,14 @@ class Catchment:
         v = C.c_double()
         self._check(lib().mfsv_volume(self._h, C.byref(v)))
         return v.value
+
+    def set_tile_rows(self, rows):
+        self._check(lib().mfsv_set_tile_rows(self._h, int(rows)))
+
+    def tile_rows(self):
+        r = C.c_int()
+        self._check(lib().mfsv_get_tile_rows(self._h, C.byref(r)))
+        return r.value
 
     def device_bytes(self):
         b = C.c_int64()

@@ -67,8 +67,9 @@ typedef struct or_cellv {
 } or_cellv;
 
 /* Drifting Gaussian storm (extension E4): rate in m/s at cell centre (x, y)
- * and global time t is peak * exp(-r^2 / (2 sigma^2)), r the distance from
- * (cx0 + vx t, cy0 + vy t). */
+ * and global time t is peak * exp(-dx^2/(2 sigma^2)) * exp(-dy^2/(2 sigma^2))
+ * with (dx, dy) the offset from (cx0 + vx t, cy0 + vy t); exp from
+ * or_exp_neg. */
 typedef struct or_storm {
   double peak, sigma, cx0, cy0, vx, vy;
 } or_storm;
@@ -86,6 +87,7 @@ void or_slope_y(int64_t nx, int64_t ny, double dy, const double* f, double* out,
                 int flags);
 double or_storm_rate(const or_storm* s, double x, double y, double t);
 double or_pow43(double x);
+double or_exp_neg(double x);
 
 /* --- whole-solver restatement ------------------------------------------ */
 /* tableau: (nstages+1) x (nstages+1) row-major, rows as the reference's a_

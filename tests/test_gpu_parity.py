@@ -47,6 +47,8 @@ def _oracle(sc):
 
 @pytest.mark.parametrize("nx,ny,flags,seed", [
     (64, 48, 0, 1),
+    (70, 40, 0, 11),           # power-of-two spacing: division by multiplication
+    (50, 30, S.E7_Y_ORIENT | S.E8_DRY_VELOCITY | S.E9_VELOCITY_CAP, 12),
     (61, 17, 0, 2),            # ragged: one column past a strip, rows not a tile multiple
     (130, 33, S.E7_Y_ORIENT, 3),
     (7, 5, S.E2_WEST_OUTFLOW, 4),
@@ -57,7 +59,8 @@ def test_dudt_matches_oracle(gpu_lib, nx, ny, flags, seed):
     from mfcm_amd.mfsv import Catchment
     _say(f"dudt nx={nx} ny={ny} flags={flags} seed={seed}")
     zb, h, u, v = _random_state(nx, ny, seed)
-    sc = _scenario_from_state(nx, ny, 5.0, 3.0, zb, h, u, v, flags=flags, rain=2e-5,
+    dx, dy = (2.0, 4.0) if seed == 11 else (5.0, 3.0)
+    sc = _scenario_from_state(nx, ny, dx, dy, zb, h, u, v, flags=flags, rain=2e-5,
                               head_cols=1 if seed % 2 else 0)
     sim = _oracle(sc)
     sim.start_new_step(30.0)
