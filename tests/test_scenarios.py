@@ -1,0 +1,41 @@
+"""Seeded scenario generators: byte-exact per seed and within spec ranges."""
+import json
+import os
+
+import numpy as np
+
+from mfcm_amd import scenarios as S
+from tests.golden.make_scenario_digests import digests
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_digests_match_golden():
+    with open(os.path.join(HERE, "golden", "scenario_digests.json")) as fh:
+        want = json.load(fh)
+    assert digests() == want
+
+
+def test_config0_terrain_properties():
+    sc = S.config0(seed=0, n=128)
+    zb = sc.zb
+    # tilted plane rising along x: the x = 0 column is the low edge on average
+    assert zb[:, :8].mean() < zb[:, -8:].mean()
+    assert sc.rain_v[0] == 50.0 * S.MM_PER_H
+    assert sc.rain_t[-1] > sc.t_end + sc.step_duration
+
+
+def test_config1_ranges():
+    sc = S.config1(seed=1, n=128)
+    assert sc.zb.min() > 290.0                         # +300 m base minus fBm
+    assert set(np.unique(sc.n_manning)) <= {0.02, 0.04, 0.06, 0.08}
+    assert sc.storm.peak == 80.0 * S.MM_PER_H and sc.storm.sigma == 400.0
+    assert abs(np.hypot(sc.storm.vx, sc.storm.vy) - 5.0) < 1e-12
+
+
+def test_ensemble_member_ranges():
+    for seed in range(20):
+        m = S.ensemble_member(seed)
+        assert 128 <= m.nx <= 1024 and 128 <= m.ny <= 1024
+        assert 2.0 <= m.dx <= 10.0
+        assert 0.02 <= m.n_manning <= 0.06
