@@ -187,6 +187,28 @@ int mfsv_last_run_ms(mfsv_ctx* ctx, double* ms);
  * event intervals around the stage launches, and the number of launches. */
 int mfsv_last_stage_ms(mfsv_ctx* ctx, double* ms, int64_t* launches);
 
+/* ---- batches: many catchments advanced per launch.  No reference
+ * counterpart (the reference runs one mesh per process, mfcm.cpp:38-49); the
+ * batch drives the same per-catchment TemporalScheme loop as mfsv_run for
+ * every member, with one persistent stage launch per RK stage over the
+ * concatenated tile tables and one controller launch for all members. ---- */
+typedef struct mfsv_batch mfsv_batch;
+int mfsv_batch_create(int precision, int device, mfsv_batch** out);
+int mfsv_batch_destroy(mfsv_batch* batch);
+const char* mfsv_batch_last_error(const mfsv_batch* batch);
+/* The member must have the batch's precision, device and RK scheme and must
+ * have been started with mfsv_run_begin.  The batch does not own it; its
+ * fields and statistics stay readable through the member's own calls. */
+int mfsv_batch_add(mfsv_batch* batch, mfsv_ctx* member);
+int mfsv_batch_remove(mfsv_batch* batch, mfsv_ctx* member);
+/* members: added and not removed; active: members not yet finished. */
+int mfsv_batch_size(const mfsv_batch* batch, int* members, int* active);
+/* Advance every active member by up to max_timesteps attempted timesteps
+ * (graph != 0: replay hipGraph-captured chunks).  Finished members leave the
+ * tile table; their results stay in their contexts. */
+int mfsv_batch_run(mfsv_batch* batch, int64_t max_timesteps, int graph);
+int mfsv_batch_last_run_ms(const mfsv_batch* batch, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

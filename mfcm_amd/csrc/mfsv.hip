@@ -55,6 +55,7 @@ struct mfsv_ctx {
   int cur = 0;
   bool nfield_set = false;
   bool nanbed = false;  // the bed has coded-out (NaN) cells
+  bool begun = false;   // mfsv_run_begin has initialised the control block
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int64_t graph_steps = 0;
@@ -255,7 +256,7 @@ int launch_stage(mfsv_ctx* c, int s, bool kout, bool branches) {
   return MFSV_OK;
 }
 
-int launch_ctrl(mfsv_ctx* c, int mode) {
+CtrlArgs ctrl_args(const mfsv_ctx* c, int mode) {
   CtrlArgs A{};
   A.ctrl = tp<Ctrl>(c->ctrl);
   A.part_vol = tp<double>(c->part_vol);
@@ -267,6 +268,11 @@ int launch_ctrl(mfsv_ctx* c, int mode) {
   A.area = c->dx * c->dy;
   A.forcing = c->nforcing ? tp<double>(c->forcing) : nullptr;
   A.nforcing = c->nforcing;
+  return A;
+}
+
+int launch_ctrl(mfsv_ctx* c, int mode) {
+  const CtrlArgs A = ctrl_args(c, mode);
   hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(256), 0, c->stream, A);
   HIPCHK(c, hipGetLastError());
   return MFSV_OK;
@@ -543,6 +549,11 @@ int mfsv_upload(mfsv_ctx* c, int field, const double* host, int64_t n) {
   if (!c || !host || !valid_field(field)) return MFSV_ERR_ARG;
   if (n != c->nc) return fail(c, MFSV_ERR_ARG, "count must equal nx*ny");
   if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  if (field <= MFSV_FIELD_V) {  // the device controller (or a batch) may have flipped U0
+    int rc = read_ctrl(c);
+    if (rc) return rc;
+    c->cur = c->hctrl->cur;
+  }
   switch (field) {
     case MFSV_FIELD_H: case MFSV_FIELD_U: case MFSV_FIELD_V:
       return put(c, c->U[c->cur][field], host, n, 0.0);
@@ -584,6 +595,11 @@ int mfsv_download(mfsv_ctx* c, int field, double* host, int64_t n) {
     return get(c, b, host, n, 0.0, 4);
   }
   if (n != c->nc) return fail(c, MFSV_ERR_ARG, "count must equal nx*ny");
+  if (field <= MFSV_FIELD_V) {
+    int rc = read_ctrl(c);
+    if (rc) return rc;
+    c->cur = c->hctrl->cur;
+  }
   switch (field) {
     case MFSV_FIELD_H: case MFSV_FIELD_U: case MFSV_FIELD_V:
       return get(c, c->U[c->cur][field], host, n, 0.0, c->vsz);
@@ -688,6 +704,7 @@ int mfsv_run_begin(mfsv_ctx* c) {
   h.cur = c->cur;
   h.done = ns <= 0 ? 1 : 0;
   h.repeat = 0;
+  c->begun = true;
   if (c->nforcing) {
     std::vector<double> f0(4);
     HIPCHK(c, hipMemcpy(f0.data(), c->forcing.p, 32, hipMemcpyDeviceToHost));
@@ -805,3 +822,5 @@ int mfsv_last_stage_ms(mfsv_ctx* c, double* ms, int64_t* launches) {
 }
 
 }  // extern "C"
+
+#include "mfsv_batch.hpp"

@@ -34,8 +34,7 @@ __device__ __forceinline__ double block_sum(const double* p, int64_t n, double* 
 
 // Controller (RungeKutta.cpp:154-167) + accept (RungeKutta.hpp:428-431) +
 // landing rules and outer-step transitions (TemporalScheme.hpp:112-149).
-__global__ __launch_bounds__(256) void k_ctrl(const CtrlArgs A) {
-  __shared__ double sh[256];
+__device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
   Ctrl* C = A.ctrl;
   if (C->done) return;
   const double vol = block_sum(A.part_vol, A.ntiles, sh) * A.area;
@@ -87,6 +86,20 @@ __global__ __launch_bounds__(256) void k_ctrl(const CtrlArgs A) {
   } else if (tl + 2.0 * ndt > sd) {
     C->dt = (sd - tl) * 0.4;
   }
+}
+
+__global__ __launch_bounds__(256) void k_ctrl(const CtrlArgs A) {
+  __shared__ double sh[256];
+  ctrl_body(A, sh);
+}
+
+// One block per member of a batch; the member count is read from device
+// memory so that captured graphs survive changes of membership.
+__global__ __launch_bounds__(256) void k_ctrl_batch(const CtrlArgs* __restrict__ args,
+                                                    const int* __restrict__ nmembers_p) {
+  __shared__ double sh[256];
+  if ((int)blockIdx.x >= *nmembers_p) return;
+  ctrl_body(args[blockIdx.x], sh);
 }
 
 // Host-driven mode: set the attempt's t_local / dt (and the buffer flip of an

@@ -147,11 +147,8 @@ __device__ __forceinline__ float vdiv(float a, float b) { return a * __builtin_a
 // QUICK: divide by dx, dy with one multiplication by the reciprocal (exact
 // for power-of-two spacings; the FP32 form always does).
 template <typename T, int NACC, bool KOUT, bool QUICK>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArgs<T> A) {
+__device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile, const int lane) {
   constexpr bool kRel = sizeof(T) == 4;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int tile = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
-  if (tile >= A.ntiles) return;
   Ctrl* __restrict__ C = A.ctrl;
   if (C->done) return;
 
@@ -516,6 +513,36 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArg
       A.part_vol[tile] = vol;
       atomicMax(&C->cn_bits, (unsigned long long)__double_as_longlong(cnmax));
     }
+  }
+}
+
+// One catchment: one wavefront per tile.
+template <typename T, int NACC, bool KOUT, bool QUICK>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArgs<T> A) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int tile = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  if (tile >= A.ntiles) return;
+  stage_tile<T, NACC, KOUT, QUICK>(A, tile, lane);
+}
+
+// Many catchments in one launch: a persistent grid walks a tile table of
+// (member, tile) pairs; member m's arguments are margs[m].  Each tile runs
+// exactly the code of the single-catchment kernel, so a catchment's result
+// does not depend on which other catchments share the launch.  The table
+// length is read from device memory so that captured graphs survive changes
+// of membership.
+template <typename T, int NACC, bool QUICK>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
+    const StageArgs<T>* __restrict__ margs, const int2* __restrict__ tiles,
+    const int* __restrict__ ntiles_p) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  const int ntiles = *ntiles_p;
+  for (int k = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6); k < ntiles;
+       k += nwaves) {
+    const int m = __builtin_amdgcn_readfirstlane(tiles[k].x);
+    const int t = __builtin_amdgcn_readfirstlane(tiles[k].y);
+    stage_tile<T, NACC, false, QUICK>(margs[m], t, lane);
   }
 }
 

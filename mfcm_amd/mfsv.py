@@ -114,6 +114,14 @@ SIGNATURES = {
     "mfsv_get_stats": (_I, [_P, C.POINTER(Stats)]),
     "mfsv_last_run_ms": (_I, [_P, C.POINTER(C.c_double)]),
     "mfsv_last_stage_ms": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "mfsv_batch_create": (_I, [_I, _I, C.POINTER(_P)]),
+    "mfsv_batch_destroy": (_I, [_P]),
+    "mfsv_batch_last_error": (C.c_char_p, [_P]),
+    "mfsv_batch_add": (_I, [_P, _P]),
+    "mfsv_batch_remove": (_I, [_P, _P]),
+    "mfsv_batch_size": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mfsv_batch_run": (_I, [_P, _I64, _I]),
+    "mfsv_batch_last_run_ms": (_I, [_P, C.POINTER(C.c_double)]),
 }
 
 _lib = None
