@@ -50,7 +50,7 @@ def log(*a):
 
 def make_scenario(n):
     from mfcm_amd import scenarios as S
-    sc = S.config1(seed=1, n=n, dx=2.0, t_end=7200.0)
+    sc = S.storm_catchment(seed=1, n=n, dx=2.0, t_end=7200.0)
     sc.flags |= S.E7_Y_ORIENT
     return sc
 
@@ -131,6 +131,150 @@ def accuracy(sc0, horizon):
     }
 
 
+def ensemble(n, steps, warmup, horizon_members, horizon):
+    """Batched ensemble: n catchments (sides log-uniform in [128, 1024], dx
+    U[2, 10] m, rain U[10, 100] mm/h for U[15, 60] min, n U[0.02, 0.06], 1 h
+    horizon each) advanced together in FP32 by the persistent tile-table
+    kernel with an autotuned tile height and graph-captured timesteps,
+    against the same catchments solved one after another by the plain FP64
+    path.  Both sides time `steps` attempted timesteps per catchment after
+    `warmup` untimed ones; cell-updates count accepted timesteps only."""
+    from mfcm_amd import mfsv
+    from mfcm_amd import scenarios as S
+    from mfcm_amd.autotune import TileTuner, model_rows_batch
+    from mfcm_amd.batch import Batch
+    scs = [S.ensemble_member(3000 + k, 128, 1024, 3600.0) for k in range(n)]
+    shapes = [(sc.nx, sc.ny) for sc in scs]
+    cells = sum(sc.ncells for sc in scs)
+    log(f"ensemble: {n} catchments, {cells / 1e6:.2f} M cells; tuning the tile height")
+    tuner = TileTuner()
+    rows = tuner.tune_batch(scs, mfsv.F32)
+    log(f"  tile rows {rows} (model {model_rows_batch(shapes, mfsv.F32)}), "
+        f"us/timestep {({k: round(v, 1) for k, v in tuner.last_timings.items()})}")
+    cats = []
+    b = Batch(mfsv.F32)
+    for sc in scs:
+        c = mfsv.Catchment(sc, precision=mfsv.F32)
+        c.set_tile_rows(rows)
+        c.run_begin()
+        b.add(c)
+        cats.append(c)
+    b.run(warmup)
+    device_sync()
+    a0 = [c.stats()["accepted"] for c in cats]
+    t0 = time.perf_counter()
+    b.run(steps)
+    device_sync()
+    wall32 = time.perf_counter() - t0
+    a1 = [c.stats()["accepted"] for c in cats]
+    cu32 = sum((y - x) * sc.ncells for x, y, sc in zip(a0, a1, scs))
+    b.close()
+    for c in cats:
+        c.close()
+    log(f"  FP32 batched: {cu32 / wall32:.4g} cell-updates/s over {wall32:.2f} s")
+    wall64, cu64 = 0.0, 0
+    for i, sc in enumerate(scs):
+        if i % 16 == 0:
+            log(f"  FP64 plain sequential: catchment {i}/{n}")
+        c = mfsv.Catchment(sc, precision=mfsv.F64)
+        c.run_begin()
+        c.run(warmup, graph=True)
+        device_sync()
+        x = c.stats()["accepted"]
+        t0 = time.perf_counter()
+        c.run(steps, graph=True)
+        device_sync()
+        wall64 += time.perf_counter() - t0
+        cu64 += (c.stats()["accepted"] - x) * sc.ncells
+        c.close()
+    log(f"  FP64 plain sequential: {cu64 / wall64:.4g} cell-updates/s over {wall64:.2f} s")
+    out = {
+        "catchments": n, "cells": cells, "tile_rows": rows,
+        "tile_timings_us_per_timestep": tuner.last_timings,
+        "timed_timesteps_per_catchment": steps,
+        "f32_batched": {"value": cu32 / wall32, "unit": "cell-updates/s", "wall_s": wall32},
+        "f64_plain_sequential": {"value": cu64 / wall64, "unit": "cell-updates/s",
+                                 "wall_s": wall64},
+        "ratio": (cu32 / wall32) / (cu64 / wall64),
+    }
+    if horizon_members:
+        errs = []
+        for k in range(min(horizon_members, n)):
+            errs.append(compare_precisions(S.ensemble_member(3000 + k, 128, 1024, horizon)))
+        out["accuracy"] = {
+            "members": horizon_members, "horizon_s": horizon,
+            "max_depth_err_m": max(e["max_depth_err_m"] for e in errs),
+            "max_velocity_err_m_s_wet": max(e["max_velocity_err_m_s_wet"] for e in errs),
+            "max_rel_mass_err_vs_f64": max(e["rel_mass_err_vs_f64"] for e in errs),
+            "max_mass_balance_residual_f32": max(abs(e["mass_balance_residual_f32"]) for e in errs),
+        }
+    return out
+
+
+def compare_precisions(sc):
+    """Run a scenario to its end in FP64 and in FP32; depth, velocity and mass
+    differences of the FP32 form."""
+    from mfcm_amd import mfsv
+    res = {}
+    for name, prec in (("f64", mfsv.F64), ("f32", mfsv.F32)):
+        cat = mfsv.Catchment(sc, precision=prec)
+        cat.run_begin()
+        st = cat.run(graph=True)
+        res[name] = cat.state() + (st,)
+        cat.close()
+    h64, u64, v64, st64 = res["f64"]
+    h32, u32, v32, st32 = res["f32"]
+    wet = h64 > 1e-3
+    V64 = float(np.sum(h64))
+    V32 = float(np.sum(h32))
+    mb = lambda s: (s["volume"] - s["volume0"] - s["source_volume"]) / max(abs(s["volume"]), 1e-300)
+    return {
+        "max_depth_err_m": float(np.abs(h32 - h64).max()),
+        "max_velocity_err_m_s_wet": float(max(np.abs(u32 - u64)[wet].max(initial=0.0),
+                                              np.abs(v32 - v64)[wet].max(initial=0.0))),
+        "rel_mass_err_vs_f64": abs(V32 - V64) / max(V64, 1e-300),
+        "mass_balance_residual_f32": mb(st32),
+    }
+
+
+def serving(rate, duration, sides, horizons, budget_gb):
+    """A Poisson stream of flood-scenario requests served by continuous
+    batching under an HBM budget, then the same requests solved one after
+    another by the plain FP64 path."""
+    from mfcm_amd import mfsv
+    from mfcm_amd.serving import FloodServer, poisson_stream, summarize
+    reqs = poisson_stream(rate, duration, seed=7, side_lo=sides[0], side_hi=sides[1],
+                          horizon_lo=horizons[0], horizon_hi=horizons[1])
+    log(f"serving: {len(reqs)} requests over {duration} s at {rate}/s, sides {sides}, "
+        f"horizons {horizons} s, budget {budget_gb} GB")
+    srv = FloodServer(mfsv.F32, hbm_budget=budget_gb * 1e9, chunk=32)
+    done = srv.serve(reqs, log=log)
+    srv.close()
+    s = summarize(done, srv.elapsed)
+    log(f"  served {s['completed']}: p50 {s['p50_s']:.2f} s, p99 {s['p99_s']:.2f} s, "
+        f"{s['cell_updates_per_s']:.4g} cell-updates/s")
+    t0 = time.perf_counter()
+    cu64 = 0
+    for i, r in enumerate(reqs):
+        if i % 10 == 0:
+            log(f"  sequential FP64: request {i}/{len(reqs)}, {time.perf_counter() - t0:.1f} s")
+        sc = r.scenario()
+        c = mfsv.Catchment(sc, precision=mfsv.F64)
+        c.run_begin()
+        st = c.run(graph=True)
+        cu64 += st["accepted"] * sc.ncells
+        c.close()
+    seq = time.perf_counter() - t0
+    log(f"  sequential FP64: {cu64 / seq:.4g} cell-updates/s over {seq:.1f} s")
+    s.update({
+        "requests": len(reqs), "rate_per_s": rate, "stream_s": duration,
+        "sides": list(sides), "horizons_s": list(horizons), "hbm_budget_gb": budget_gb,
+        "precision": "f32 (f64 time, controller and mass)",
+        "sequential_f64_plain": {"value": cu64 / seq, "unit": "cell-updates/s", "wall_s": seq},
+    })
+    return s
+
+
 def cpu_baseline(sc, budget_s):
     """The CPU oracle on the first timesteps of the same workload."""
     from oracle import oracle as O
@@ -177,6 +321,18 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=64, help="catchments in the batched ensemble")
+    ap.add_argument("--ensemble-steps", type=int, default=256)
+    ap.add_argument("--ensemble-accuracy", type=int, default=8,
+                    help="members compared FP32 vs FP64 to a common horizon")
+    ap.add_argument("--ensemble-horizon", type=float, default=600.0)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="requests per second")
+    ap.add_argument("--serve-duration", type=float, default=20.0, help="seconds of arrivals")
+    ap.add_argument("--serve-sides", type=str, default="128,512")
+    ap.add_argument("--serve-horizons", type=str, default="300,900")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB of a plain device copy to run first (counter calibration)")
     args = ap.parse_args()
@@ -221,6 +377,16 @@ def main():
     if not args.no_accuracy:
         log(f"accuracy FP32 vs FP64 over {args.accuracy_horizon} s")
         acc = accuracy(sc, args.accuracy_horizon)
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble(args.ensemble, args.ensemble_steps, 32, args.ensemble_accuracy,
+                       args.ensemble_horizon)
+    srv = None
+    if not args.no_serve:
+        srv = serving(args.serve_rate, args.serve_duration,
+                      tuple(int(x) for x in args.serve_sides.split(",")),
+                      tuple(float(x) for x in args.serve_horizons.split(",")),
+                      args.serve_budget_gb)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -250,7 +416,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (f64 time, controller and mass)",
         "data": "synthetic",
-        "config": {"workload": f"config1 single catchment {args.size}x{args.size} dx=2m, "
+        "config": {"workload": f"storm_catchment: single catchment {args.size}x{args.size} dx=2m, "
                                f"30 hills + fBm 1m on +300m, Voronoi n, drifting storm, "
                                f"heun2, CFL 0.5, flags E1|E2|E4|E7|E8|E9, dry start",
                    "cells": N, "precision_form": "f32"},
@@ -263,6 +429,8 @@ def main():
         "timesteps": {k: {"accepted": res[k]["accepted"], "rejected": res[k]["rejected"],
                           "sim_time_s": res[k]["sim_time_s"]} for k in res},
         "accuracy": acc,
+        "ensemble": ens,
+        "serving": srv,
     }
     print(json.dumps(line), flush=True)
 

@@ -1,0 +1,149 @@
+"""Tile-height autotuner for the stage kernel.
+
+The stage kernel gives each wavefront a strip of 60 interior columns and
+``rows`` interior rows (plus a 2-row halo above and below).  Fewer rows mean
+more halo work per cell; more rows mean fewer wavefronts, which can leave
+SIMDs idle or unevenly loaded (4480 waves on 1024 SIMDs leave a tail).  The
+best height depends on the grid shape and the precision, so it is measured:
+``TileTuner.tune`` times each candidate on a scratch copy of the catchment
+(HIP events around the stage launches) and caches the winner keyed by
+(precision, nx, ny).  Results never depend on the choice: every cell's
+arithmetic is the same for any tiling.
+
+Without a measurement ``rows_for`` uses a model of the same trade-off.
+"""
+from __future__ import annotations
+
+import copy
+import json
+import math
+import os
+
+CANDIDATES = (8, 12, 16, 24, 32)
+STRIP = 60
+SIMDS = 1024          # 256 CUs x 4 SIMDs on MI355X
+OCCUPANCY = {0: 2, 1: 4}   # waves per SIMD of the stage kernel: F64, F32
+
+
+def model_rows(nx: int, ny: int, precision: int) -> int:
+    """Candidate with the lowest modelled time: halo overhead (rows + 2)/rows
+    times the wave-count quantisation ceil(w)/w over full occupancy rounds."""
+    occ = OCCUPANCY.get(precision, 2)
+    strips = math.ceil(nx / STRIP)
+    best, best_cost = CANDIDATES[0], float("inf")
+    for r in CANDIDATES:
+        waves = strips * math.ceil(ny / r)
+        rounds = waves / (SIMDS * occ)
+        quant = math.ceil(rounds) / rounds if rounds > 0 else 1.0
+        cost = (r + 2.0) / r * quant
+        if cost < best_cost - 1e-12:
+            best, best_cost = r, cost
+    return best
+
+
+def model_rows_batch(shapes, precision: int) -> int:
+    """One tile height for a batch of (nx, ny) grids sharing a persistent
+    launch: the same trade-off over the concatenated tile table, where the
+    halo overhead is weighted by each grid's cells and the last strip of a
+    grid wastes the rows beyond ny."""
+    occ = OCCUPANCY.get(precision, 2)
+    cells = sum(nx * ny for nx, ny in shapes)
+    if cells == 0:
+        return CANDIDATES[0]
+    best, best_cost = CANDIDATES[0], float("inf")
+    for r in CANDIDATES:
+        waves = sum(math.ceil(nx / STRIP) * math.ceil(ny / r) for nx, ny in shapes)
+        work = sum(math.ceil(nx / STRIP) * STRIP * math.ceil(ny / r) * (r + 2)
+                   for nx, ny in shapes)
+        rounds = waves / (SIMDS * occ)
+        quant = math.ceil(rounds) / rounds if rounds > 0 else 1.0
+        cost = work / cells * quant
+        if cost < best_cost - 1e-12:
+            best, best_cost = r, cost
+    return best
+
+
+class TileTuner:
+    def __init__(self, cache_path: str | None = None):
+        self.cache_path = cache_path
+        self.table: dict[str, int] = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as fh:
+                self.table = {k: int(v) for k, v in json.load(fh).items()}
+
+    @staticmethod
+    def key(precision, nx, ny):
+        return f"{int(precision)}:{int(nx)}x{int(ny)}"
+
+    def rows_for(self, nx, ny, precision) -> int:
+        return self.table.get(self.key(precision, nx, ny), model_rows(nx, ny, precision))
+
+    def tune(self, scenario, precision, warmup=20, samples=20, candidates=CANDIDATES) -> int:
+        """Time every candidate on a scratch catchment of the same scenario."""
+        from .mfsv import Catchment
+        sc = copy.copy(scenario)
+        cat = Catchment(sc, precision=precision)
+        try:
+            cat.run_begin()
+            cat.run(warmup, graph=True)
+            best, best_us = None, float("inf")
+            timings = {}
+            for r in candidates:
+                cat.set_tile_rows(r)
+                cat.run(samples, graph=2)
+                ms, n = cat.last_stage_ms()
+                if n == 0:
+                    continue
+                us = ms / n * 1e3
+                timings[r] = us
+                if us < best_us:
+                    best, best_us = r, us
+        finally:
+            cat.close()
+        if best is None:
+            best = model_rows(scenario.nx, scenario.ny, precision)
+        self.table[self.key(precision, scenario.nx, scenario.ny)] = best
+        self.last_timings = timings
+        return best
+
+    def rows_for_batch(self, shapes, precision) -> int:
+        key = f"{int(precision)}:batch{len(shapes)}:{sum(nx * ny for nx, ny in shapes)}"
+        return self.table.get(key, model_rows_batch(shapes, precision))
+
+    def tune_batch(self, scenarios, precision, warmup=16, samples=48,
+                   candidates=CANDIDATES) -> int:
+        """Time a shared tile height for a batch: scratch copies of the members
+        run `samples` timesteps per candidate (the batch's event-timed run).
+        The members advance while they are timed, so a candidate is judged on
+        a state a little later than the previous one; the scratch states are
+        discarded."""
+        from .batch import Batch
+        from .mfsv import Catchment
+        cats = [Catchment(copy.copy(sc), precision=precision) for sc in scenarios]
+        b = Batch(precision)
+        timings = {}
+        try:
+            for c in cats:
+                c.run_begin()
+                b.add(c)
+            b.run(warmup)
+            for r in candidates:
+                for c in cats:
+                    c.set_tile_rows(r)
+                b.run(samples)
+                timings[r] = b.last_run_ms() / samples * 1e3
+        finally:
+            b.close()
+            for c in cats:
+                c.close()
+        shapes = [(sc.nx, sc.ny) for sc in scenarios]
+        best = min(timings, key=timings.get) if timings else model_rows_batch(shapes, precision)
+        key = f"{int(precision)}:batch{len(shapes)}:{sum(nx * ny for nx, ny in shapes)}"
+        self.table[key] = best
+        self.last_timings = timings
+        return best
+
+    def save(self):
+        if self.cache_path:
+            with open(self.cache_path, "w") as fh:
+                json.dump(self.table, fh, indent=1, sort_keys=True)

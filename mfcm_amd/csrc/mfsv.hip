@@ -56,6 +56,8 @@ struct mfsv_ctx {
   bool nfield_set = false;
   bool nanbed = false;  // the bed has coded-out (NaN) cells
   bool begun = false;   // mfsv_run_begin has initialised the control block
+  mfsv_batch* batch = nullptr;  // the batch this context is a member of
+  uint64_t gen = 0;     // bumped by every configuration change
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int64_t graph_steps = 0;
@@ -105,7 +107,11 @@ void release(Buf& b) {
   b.bytes = 0;
 }
 
+// Every configuration change (parameters, scheme, tile height, bed,
+// forcing table) passes through here: the captured graph is stale, and so
+// are the launch tables of a batch holding this context (it compares gen).
 void drop_graph(mfsv_ctx* c) {
+  ++c->gen;
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->gexec = nullptr;
@@ -426,6 +432,7 @@ int mfsv_create(const mfsv_grid* g, int precision, int device, mfsv_ctx** out) {
 
 int mfsv_destroy(mfsv_ctx* c) {
   if (!c) return MFSV_ERR_ARG;
+  if (c->batch) (void)mfsv_batch_remove(c->batch, c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   for (auto& a : c->U) for (auto& b : a) release(b);
@@ -517,6 +524,7 @@ int mfsv_set_time(mfsv_ctx* c, double start, double end, double step_duration, d
 
 int mfsv_start_new_step(mfsv_ctx* c, double time_now, double q0, double q1, double hb0, double hb1) {
   if (!c) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
   if (read_ctrl(c)) return MFSV_ERR_HIP;
   Ctrl& h = *c->hctrl;
   h.q0 = q0; h.q1 = q1; h.hb0 = hb0; h.hb1 = hb1;
@@ -625,6 +633,7 @@ static int check_inputs(mfsv_ctx* c) {
 
 int mfsv_dudt(mfsv_ctx* c, double t_sub, double dt, int want_branches) {
   if (!c || !(dt > 0)) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
   if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
   int rc;
   if ((rc = check_inputs(c))) return rc;
@@ -643,6 +652,7 @@ int mfsv_dudt(mfsv_ctx* c, double t_sub, double dt, int want_branches) {
 
 int mfsv_timestep(mfsv_ctx* c, double t_local, double dt, double* cn) {
   if (!c || !(dt > 0) || !cn) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
   if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
   int rc;
   if ((rc = check_inputs(c))) return rc;
@@ -655,6 +665,7 @@ int mfsv_timestep(mfsv_ctx* c, double t_local, double dt, double* cn) {
 
 int mfsv_accept(mfsv_ctx* c) {
   if (!c) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
   int rc = set_ctrl(c, 0, 0, 1, 0);
   if (rc) return rc;
   c->cur ^= 1;
@@ -679,6 +690,7 @@ static int64_t num_steps(const mfsv_ctx* c) {
 
 int mfsv_run_begin(mfsv_ctx* c) {
   if (!c) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
   if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
   int rc;
   if ((rc = check_inputs(c))) return rc;
@@ -737,6 +749,7 @@ static int build_graph(mfsv_ctx* c, int64_t steps) {
 
 int mfsv_run(mfsv_ctx* c, int64_t max_timesteps, int use_graph) {
   if (!c || max_timesteps < 0) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
   if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
   int rc;
   const int64_t batch = 32;

@@ -16,6 +16,7 @@ struct mfsv_batch {
   double tab[kMaxStages + 1][kMaxStages + 1] = {};
   std::vector<mfsv_ctx*> members;  // in insertion order
   std::vector<mfsv_ctx*> active;   // members in the current tables
+  std::vector<uint64_t> gens;      // active members' configuration generations
   Buf margs[kMaxStages];           // StageArgs<T>[cap] per stage
   Buf cargs;                       // CtrlArgs[cap]
   Buf tiles;                       // int2[tile_cap]
@@ -138,6 +139,8 @@ int upload_tables(mfsv_batch* b) {
   BHIPCHK(b, hipMemcpyAsync(b->counts.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, b->stream));
   BHIPCHK(b, hipStreamSynchronize(b->stream));
   b->ntiles = nt;
+  b->gens.clear();
+  for (auto* c : b->active) b->gens.push_back(c->gen);
   b->dirty = false;
   return MFSV_OK;
 }
@@ -203,6 +206,7 @@ int mfsv_batch_create(int precision, int device, mfsv_batch** out) {
 int mfsv_batch_destroy(mfsv_batch* b) {
   if (!b) return MFSV_ERR_ARG;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto* c : b->members) c->batch = nullptr;
   bdrop_graph(b);
   for (auto& m : b->margs) release(m);
   for (Buf* p : {&b->cargs, &b->tiles, &b->counts, &b->done}) release(*p);
@@ -226,8 +230,8 @@ int mfsv_batch_add(mfsv_batch* b, mfsv_ctx* c) {
   } else if (c->nstages != b->nstages || std::memcmp(b->tab, c->tab, sizeof(b->tab)) != 0) {
     return bfail(b, MFSV_ERR_ARG, "member Runge-Kutta scheme differs from the batch");
   }
-  for (auto* m : b->members)
-    if (m == c) return bfail(b, MFSV_ERR_STATE, "member already in the batch");
+  if (c->batch == b) return bfail(b, MFSV_ERR_STATE, "member already in the batch");
+  if (c->batch) return bfail(b, MFSV_ERR_STATE, "member of another batch");
   if (!c->begun) return bfail(b, MFSV_ERR_STATE, "call mfsv_run_begin on the member first");
   int rc = check_inputs(c);
   if (rc) return bfail(b, rc, c->err);
@@ -235,6 +239,7 @@ int mfsv_batch_add(mfsv_batch* b, mfsv_ctx* c) {
   BHIPCHK(b, hipStreamSynchronize(c->stream));
   b->members.push_back(c);
   b->active.push_back(c);
+  c->batch = b;
   b->dirty = true;
   return MFSV_OK;
 }
@@ -245,6 +250,7 @@ int mfsv_batch_remove(mfsv_batch* b, mfsv_ctx* c) {
   if (it == b->members.end()) return bfail(b, MFSV_ERR_STATE, "not a member");
   BHIPCHK(b, hipStreamSynchronize(b->stream));
   b->members.erase(it);
+  c->batch = nullptr;
   auto ja = std::find(b->active.begin(), b->active.end(), c);
   if (ja != b->active.end()) b->active.erase(ja);
   b->dirty = true;
@@ -263,6 +269,16 @@ int mfsv_batch_run(mfsv_batch* b, int64_t max_timesteps, int use_graph) {
   if (hipSetDevice(b->dev) != hipSuccess) return bfail(b, MFSV_ERR_HIP, "hipSetDevice");
   int rc;
   const int64_t chunk = 16;
+  // a member reconfigured since the tables were built (tile height,
+  // parameters, bed, forcing) invalidates them
+  if (!b->dirty) {
+    if (b->gens.size() != b->active.size()) b->dirty = true;
+    for (size_t j = 0; j < b->active.size() && !b->dirty; ++j)
+      b->dirty = b->active[j]->gen != b->gens[j];
+  }
+  for (auto* c : b->active)
+    if (c->nstages != b->nstages || std::memcmp(b->tab, c->tab, sizeof(b->tab)) != 0)
+      return bfail(b, MFSV_ERR_STATE, "a member's Runge-Kutta scheme changed after it joined");
   BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
   int64_t left = max_timesteps;
   while (left > 0) {

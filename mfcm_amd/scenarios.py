@@ -170,7 +170,7 @@ def rain_series(rate_m_s: float, duration_s: float, t_end: float, step: float):
     return t, v
 
 
-def config0(seed=0, n=256, dx=5.0, prewet=0.0, physics=False, t_end=3600.0):
+def hillslope_rain(seed=0, n=256, dx=5.0, prewet=0.0, physics=False, t_end=3600.0):
     """256x256 tilted plane + 8 hills + fBm 0.5 m, rain 50 mm/h for 30 min,
     n = 0.03, Heun-2, 1 h.  Parity mode (physics=False): the reference's
     semantics with a head(0) sink on column x = 0 as the outflow.  Physics mode:
@@ -179,7 +179,7 @@ def config0(seed=0, n=256, dx=5.0, prewet=0.0, physics=False, t_end=3600.0):
     h0 = np.full((n, n), float(prewet))
     z = np.zeros((n, n))
     rt, rv = rain_series(50.0 * MM_PER_H, 1800.0, t_end, 60.0)
-    sc = Scenario("config0", n, n, dx, dx, zb, h0, z.copy(), z.copy(), 0.03,
+    sc = Scenario("hillslope_rain", n, n, dx, dx, zb, h0, z.copy(), z.copy(), 0.03,
                   rain_t=rt, rain_v=rv, t_end=t_end)
     if physics:
         sc.flags = PHYSICS
@@ -189,7 +189,7 @@ def config0(seed=0, n=256, dx=5.0, prewet=0.0, physics=False, t_end=3600.0):
     return sc
 
 
-def config1(seed=1, n=2048, dx=2.0, t_end=7200.0, prewet=0.0):
+def storm_catchment(seed=1, n=2048, dx=2.0, t_end=7200.0, prewet=0.0):
     """2048^2, dx 2 m, 30 hills + fBm 1 m on +300 m base; drifting storm
     (80 mm/h peak, sigma 400 m, 5 m/s); Voronoi Manning n in [0.02, 0.08];
     2 h.  Physics mode flags E1 | E2 | E4, CFL 0.5."""
@@ -197,13 +197,13 @@ def config1(seed=1, n=2048, dx=2.0, t_end=7200.0, prewet=0.0):
     nm = voronoi_classes(seed + 1000, n, n)
     st = drifting_storm(seed + 2000, n, n, dx)
     z = np.zeros((n, n))
-    sc = Scenario("config1", n, n, dx, dx, zb, np.full((n, n), float(prewet)), z.copy(),
+    sc = Scenario("storm_catchment", n, n, dx, dx, zb, np.full((n, n), float(prewet)), z.copy(),
                   z.copy(), nm, storm=st, t_end=t_end,
                   flags=PHYSICS | E4_STORM, cn_target=0.5)
     return sc
 
 
-def config2(seed=2, n=4096, dx=1.0, t_end=600.0, wall_x=1000, breach=50):
+def breach_front(seed=2, n=4096, dx=1.0, t_end=600.0, wall_x=1000, breach=50):
     """4096^2, dx 1 m, flat bed + fBm 0.2 m, a 1-cell wall column at x = wall_x
     (bed +10 m) with a breach of `breach` cells, 5 m upstream, dry downstream,
     n = 0.025, no rain, 600 s."""
@@ -216,13 +216,13 @@ def config2(seed=2, n=4096, dx=1.0, t_end=600.0, wall_x=1000, breach=50):
     h0 = np.zeros((n, n))
     h0[:, :wall_x] = 5.0
     z = np.zeros((n, n))
-    return Scenario("config2", n, n, dx, dx, np.ascontiguousarray(zb), h0, z.copy(), z.copy(),
+    return Scenario("breach_front", n, n, dx, dx, np.ascontiguousarray(zb), h0, z.copy(), z.copy(),
                     0.025, t_end=t_end, cn_target=0.5, flags=E7_Y_ORIENT | E8_DRY_VELOCITY)
 
 
 def ensemble_member(seed: int, side_lo=128, side_hi=1024, t_end=3600.0):
     """One catchment of the batched ensemble: sides log-uniform per axis in
-    [side_lo, side_hi], dx U[2, 10], config-0 terrain, rain U[10, 100] mm/h for
+    [side_lo, side_hi], dx U[2, 10], hillslope_rain terrain, rain U[10, 100] mm/h for
     U[15, 60] min, n U[0.02, 0.06]."""
     rng = np.random.default_rng(seed)
     nx = int(round(math.exp(rng.uniform(math.log(side_lo), math.log(side_hi)))))

@@ -13,9 +13,9 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from mfcm_amd import scenarios as S  # noqa: E402
 
 CASES = {
-    "config0_n64_seed0": lambda: S.config0(seed=0, n=64),
-    "config1_n96_seed1": lambda: S.config1(seed=1, n=96),
-    "config2_n128_seed2": lambda: S.config2(seed=2, n=128, wall_x=40, breach=10),
+    "hillslope_rain_n64_seed0": lambda: S.hillslope_rain(seed=0, n=64),
+    "storm_catchment_n96_seed1": lambda: S.storm_catchment(seed=1, n=96),
+    "breach_front_n128_seed2": lambda: S.breach_front(seed=2, n=128, wall_x=40, breach=10),
     "member_seed5": lambda: S.ensemble_member(5, side_lo=32, side_hi=96),
     "dambreak_64x48": lambda: S.dam_break_bump(64, 48),
 }

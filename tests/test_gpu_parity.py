@@ -92,8 +92,8 @@ def _compare_states(cat, sim, tol=1e-9):
     return np.abs(hg - ho).max()
 
 
-@pytest.mark.parametrize("case", ["dambreak_parity", "dambreak_e7", "config0_prewet_parity",
-                                  "config0_physics", "config1_physics"])
+@pytest.mark.parametrize("case", ["dambreak_parity", "dambreak_e7", "hillslope_rain_prewet_parity",
+                                  "hillslope_rain_physics", "storm_catchment_physics"])
 def test_run_matches_oracle(gpu_lib, case):
     from mfcm_amd.mfsv import Catchment
     if case.startswith("dambreak"):
@@ -101,12 +101,12 @@ def test_run_matches_oracle(gpu_lib, case):
         sc.t_end = 60.0 if case.endswith("parity") else 120.0
         if case.endswith("e7"):
             sc.flags |= S.E7_Y_ORIENT
-    elif case == "config1_physics":
-        sc = S.config1(seed=1, n=64, dx=2.0, t_end=180.0)
-    elif case == "config0_prewet_parity":
-        sc = S.config0(seed=0, n=64, dx=5.0, prewet=0.002, t_end=300.0)
+    elif case == "storm_catchment_physics":
+        sc = S.storm_catchment(seed=1, n=64, dx=2.0, t_end=180.0)
+    elif case == "hillslope_rain_prewet_parity":
+        sc = S.hillslope_rain(seed=0, n=64, dx=5.0, prewet=0.002, t_end=300.0)
     else:
-        sc = S.config0(seed=0, n=64, dx=5.0, physics=True, t_end=240.0)
+        sc = S.hillslope_rain(seed=0, n=64, dx=5.0, physics=True, t_end=240.0)
         sc.flags |= S.E7_Y_ORIENT
     _say(f"run {case} {sc.nx}x{sc.ny} t_end={sc.t_end} flags={sc.flags}")
     sim = _oracle(sc)
@@ -126,7 +126,7 @@ def test_run_matches_oracle(gpu_lib, case):
 def test_device_controller_equals_host_loop(gpu_lib, graph):
     """The device controller (mfsv_run) repeats the host loop bit for bit."""
     from mfcm_amd.mfsv import Catchment
-    sc = S.config0(seed=4, n=96, dx=5.0, prewet=0.002, t_end=240.0)
+    sc = S.hillslope_rain(seed=4, n=96, dx=5.0, prewet=0.002, t_end=240.0)
     _say(f"device controller graph={graph}")
     a = Catchment(sc)
     a.solve()
@@ -143,7 +143,7 @@ def test_device_controller_equals_host_loop(gpu_lib, graph):
 
 def test_device_run_matches_oracle_and_mass(gpu_lib):
     from mfcm_amd.mfsv import Catchment
-    sc = S.config0(seed=5, n=80, dx=5.0, physics=True, t_end=180.0)
+    sc = S.hillslope_rain(seed=5, n=80, dx=5.0, physics=True, t_end=180.0)
     sc.flags |= S.E7_Y_ORIENT
     _say("device run vs oracle, physics flags")
     sim = _oracle(sc)

@@ -16,8 +16,8 @@ def test_digests_match_golden():
     assert digests() == want
 
 
-def test_config0_terrain_properties():
-    sc = S.config0(seed=0, n=128)
+def test_hillslope_rain_terrain_properties():
+    sc = S.hillslope_rain(seed=0, n=128)
     zb = sc.zb
     # tilted plane rising along x: the x = 0 column is the low edge on average
     assert zb[:, :8].mean() < zb[:, -8:].mean()
@@ -25,8 +25,8 @@ def test_config0_terrain_properties():
     assert sc.rain_t[-1] > sc.t_end + sc.step_duration
 
 
-def test_config1_ranges():
-    sc = S.config1(seed=1, n=128)
+def test_storm_catchment_ranges():
+    sc = S.storm_catchment(seed=1, n=128)
     assert sc.zb.min() > 290.0                         # +300 m base minus fBm
     assert set(np.unique(sc.n_manning)) <= {0.02, 0.04, 0.06, 0.08}
     assert sc.storm.peak == 80.0 * S.MM_PER_H and sc.storm.sigma == 400.0
