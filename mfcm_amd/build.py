@@ -16,6 +16,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "mfsv.hip")
 OUT = os.path.join(HERE, "libmfsv_hip.so")
+DRIVER_SRC = os.path.join(HERE, "host", "mfsv_drive.cpp")
+DRIVER = os.path.join(HERE, "host", "mfsv_drive")
 ARCH = os.environ.get("MFSV_ARCH", "gfx950")
 
 FLAGS = [
@@ -44,7 +46,24 @@ def build(verbose: bool = False, save_asm: bool = False) -> str:
         subprocess.run([hipcc, "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract=off",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                         "--cuda-device-only", "-S", "-o", asm, SRC], check=True)
+    build_driver(verbose)
     return OUT
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The C++ mirror's command-line driver, linked against libmfsv_hip.so
+    (found next to it through the run path)."""
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", DRIVER + ".tmp", DRIVER_SRC,
+           f"-L{HERE}", "-lmfsv_hip", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building the C++ driver failed")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -1,0 +1,220 @@
+// SaintVenantSolver.hpp -- the Saint-Venant solver of the C++ mirror, over
+// the C ABI of libmfsv_hip.so.
+//
+// Mirrors SaintVenantSolver<TT, T, Mesh> (SaintVenant/Solver.hpp:30-172,
+// Solver.cpp:36-129) with the state of SaintVenantState (State.hpp:24-156:
+// primitive h, u, v), the Manning source (ManningRoughnessSourceTerm.cpp:49-109)
+// and the discharge / head boundaries (DischargeBoundarySourceTerm.hpp:114-148,
+// HeadBoundarySourceTerm.cpp:43-66) with their time series
+// (Input/TimeSeries.cpp:27-46).
+//
+// The device executes every RK stage of a timestep as one fused kernel
+// launch per stage (slopes, faces, cell pass, sources, boundaries and the RK
+// accumulation together).  The reference's per-stage calls keep their
+// meaning: update_dUdt(s, t, dt) with s = 0 opens a timestep at local time t,
+// update_y(n, dt) for the last row runs the fused stages of that timestep,
+// and control_number() returns the control number of its result
+// (State.cpp:199-229).  accept() makes that result the current state
+// (RungeKutta.hpp:428-431, a buffer flip on the device).  dUdt_of_state()
+// evaluates dU/dt of the current state alone, for inspection.
+#pragma once
+
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "DataArray.hpp"
+#include "RungeKutta.hpp"
+#include "TemporalScheme.hpp"
+
+namespace mfsv {
+
+// Piecewise-linear time series (Input/TimeSeries.cpp:27-46, 109-115).  Past
+// the last sample the reference reads beyond its arrays; here that is NaN,
+// so a series must cover [start, end + step].
+class TimeSeries {
+ public:
+  TimeSeries() = default;
+  TimeSeries(std::vector<double> t, std::vector<double> v) : t_(std::move(t)), v_(std::move(v)) {
+    if (t_.size() != v_.size() || t_.size() < 2)
+      throw std::runtime_error("TimeSeries: need matching times and values, at least 2 samples");
+  }
+  bool empty() const { return t_.empty(); }
+  double at(double time) const {
+    size_t i1 = 1;
+    while (i1 < t_.size() && !(t_[i1] > time)) ++i1;
+    if (i1 >= t_.size()) return std::numeric_limits<double>::quiet_NaN();
+    const size_t i0 = i1 - 1;
+    const double dvdt = (v_[i1] - v_[i0]) / (t_[i1] - t_[i0]);
+    return v_[i0] + dvdt * (time - t_[i0]);
+  }
+
+ private:
+  std::vector<double> t_, v_;
+};
+
+// What the reference reads from its config sections (source terms,
+// boundaries, scheme), as plain values; arrays are injected.
+struct SaintVenantOptions {
+  int precision = MFSV_F64;
+  int device = 0;
+  int32_t flags = 0;            // MFSV_E* extension bits (0 = the reference's semantics)
+  double cn_target = 1.0;
+  bool manning = true;
+  double n_shallow = 0.03, n_deep = 0.03, d_shallow = 0.1, d_deep = 0.3;
+  std::vector<double> manning_n;  // per-cell n (empty = uniform)
+  TimeSeries rain;                // discharge boundary, global selection (m/s)
+  int64_t head_cols = 0;          // head boundary on columns [0, head_cols), target depth 0
+  mfsv_params storm{};            // storm_* fields used when flags has MFSV_E4_STORM
+  double h_dry = 1e-3, u_max = 10.0;
+  int tile_rows = 0;              // 0 = library default
+};
+
+template <typename TT, typename T>
+class SaintVenantSolver {
+ public:
+  using TimeType = TT;
+  using ValueType = T;
+  using MeshType = Cartesian2DMesh;
+
+  // State (h, u, v) with host mirrors of the device fields.
+  struct State {
+    Field<double> h, u, v;  // host copies are FP64 in both precisions
+    explicit State(const Cartesian2DMesh& m) : h("h", m), u("u", m), v("v", m) {}
+  };
+
+  // Solver.cpp:36-71: mesh, bed, initial state, sources and boundaries.
+  SaintVenantSolver(const Cartesian2DMesh& mesh, const std::vector<double>& z_bed,
+                    const std::vector<double>& h0, const std::vector<double>& u0,
+                    const std::vector<double>& v0, const SaintVenantOptions& opt,
+                    size_t n_states, const std::shared_ptr<TimeParameters<TT>>& tparams,
+                    const RungeKuttaCoefficientSet& coeffs)
+      : mesh_(mesh), opt_(opt), tparams_(tparams), nstages_(n_states - 1),
+        ctx_(std::make_shared<Context>(mesh.grid(), opt.precision, opt.device)),
+        U_(mesh), zb_("z_bed", mesh), k_(mesh) {
+    const size_t n = mesh.cell_count();
+    if (z_bed.size() != n || h0.size() != n || u0.size() != n || v0.size() != n)
+      throw std::runtime_error("SaintVenantSolver: field sizes must equal the cell count");
+    if (n_states != coeffs.nsteps() + 1)
+      throw std::logic_error("SaintVenantSolver: n_states must be the scheme's stages + 1");
+    mfsv_ctx* c = ctx_->get();
+    const std::vector<double> tab = coeffs.matrix();
+    check(mfsv_set_scheme(c, int(coeffs.nsteps()), tab.data()), c, "mfsv_set_scheme");
+    mfsv_params p = opt.storm;
+    p.flags = opt.flags;
+    p.cn_target = opt.cn_target;
+    p.manning = opt.manning ? 1 : 0;
+    p.n_shallow = opt.n_shallow; p.n_deep = opt.n_deep;
+    p.d_shallow = opt.d_shallow; p.d_deep = opt.d_deep;
+    p.discharge = opt.rain.empty() ? 0 : 1;
+    p.head = opt.head_cols > 0 ? 1 : 0;
+    p.head_x0 = 0; p.head_x1 = opt.head_cols;
+    p.h_dry = opt.h_dry; p.u_max = opt.u_max;
+    check(mfsv_set_params(c, &p), c, "mfsv_set_params");
+    check(mfsv_set_time(c, tparams->start_time(), tparams->end_time(), tparams->step_duration(),
+                        tparams->initial_timestep()), c, "mfsv_set_time");
+    if (opt.tile_rows > 0) check(mfsv_set_tile_rows(c, opt.tile_rows), c, "mfsv_set_tile_rows");
+    bind_and_upload(zb_.data(), MFSV_FIELD_ZB, z_bed);
+    if (!opt.manning_n.empty()) {
+      if (opt.manning_n.size() != n) throw std::runtime_error("manning_n size != cell count");
+      DataArray<double> nf(n);
+      bind_and_upload(nf, MFSV_FIELD_MANNING_N, opt.manning_n);
+    }
+    bind_and_upload(U_.h.data(), MFSV_FIELD_H, h0);
+    bind_and_upload(U_.u.data(), MFSV_FIELD_U, u0);
+    bind_and_upload(U_.v.data(), MFSV_FIELD_V, v0);
+    k_.h.data().bind(ctx_, MFSV_FIELD_DHDT);
+    k_.u.data().bind(ctx_, MFSV_FIELD_DUDT);
+    k_.v.data().bind(ctx_, MFSV_FIELD_DVDT);
+  }
+
+  double cn_target() const { return opt_.cn_target; }
+  const Cartesian2DMesh& mesh() const { return mesh_; }
+  mfsv_ctx* context() const { return ctx_->get(); }
+
+  // Solver.cpp:77-114 for stage s; see the file comment for the fused mapping.
+  void update_dUdt(size_t s, TT substep_time, TT timestep) {
+    if (s == 0) {
+      t_local_ = substep_time;  // c_1 = 0 for every explicit tableau
+      dt_ = timestep;
+      open_ = true;
+    } else if (!open_ || timestep != dt_) {
+      throw std::logic_error("update_dUdt: stage " + std::to_string(s) + " outside a timestep");
+    }
+  }
+  // RungeKutta.hpp:371-405: rows 1..n-1 are formed inside the fused stage
+  // launches; the last row launches them.
+  void update_y(size_t step, TT timestep) {
+    if (step < nstages_) return;
+    if (!open_ || timestep != dt_) throw std::logic_error("update_y outside a timestep");
+    double cn = 0.0;
+    check(mfsv_timestep(ctx_->get(), double(t_local_), double(dt_), &cn), ctx_->get(),
+          "mfsv_timestep");
+    cn_ = cn;
+    open_ = false;
+  }
+  // State.cpp:199-229 on the result of the last timestep.
+  double control_number(size_t /*state*/, TT /*dt*/) const { return cn_; }
+  void accept() { check(mfsv_accept(ctx_->get()), ctx_->get(), "mfsv_accept"); }
+
+  // Solver.cpp:119-129: boundary endpoints for [time_now, time_now + step).
+  void start_new_step(TT time_now) {
+    double q0 = 0.0, q1 = 0.0;
+    if (!opt_.rain.empty()) {
+      q0 = 0.0 + opt_.rain.at(double(time_now));
+      q1 = 0.0 + opt_.rain.at(double(time_now + tparams_->step_duration()));
+    }
+    const double hb = opt_.head_cols > 0 ? 0.0 : std::numeric_limits<double>::quiet_NaN();
+    check(mfsv_start_new_step(ctx_->get(), double(time_now), q0, q1, hb, hb), ctx_->get(),
+          "mfsv_start_new_step");
+  }
+  void end_of_step(TT /*time_now*/) {}
+  void do_outputs(TT /*time_now*/) {}
+
+  // The current state U0, downloaded into the host mirrors.
+  State& state() {
+    U_.h.data().move_to_host();
+    U_.u.data().move_to_host();
+    U_.v.data().move_to_host();
+    return U_;
+  }
+  // dU/dt of the current state at stage time t with timestep dt.
+  State& dUdt_of_state(TT t, TT dt) {
+    check(mfsv_dudt(ctx_->get(), double(t), double(dt), 0), ctx_->get(), "mfsv_dudt");
+    k_.h.data().move_to_host();
+    k_.u.data().move_to_host();
+    k_.v.data().move_to_host();
+    return k_;
+  }
+  // FP64 volume sum(h) dx dy of the current state (Output/Measure.hpp:137-153).
+  double volume() const {
+    double v = 0.0;
+    check(mfsv_volume(ctx_->get(), &v), ctx_->get(), "mfsv_volume");
+    return v;
+  }
+
+ private:
+  void bind_and_upload(DataArray<double>& a, int field, const std::vector<double>& values) {
+    a = DataArray<double>(values.size());
+    for (size_t i = 0; i < values.size(); ++i) a[i] = values[i];
+    a.bind(ctx_, field);
+    a.move_to_device();
+  }
+
+  Cartesian2DMesh mesh_;
+  SaintVenantOptions opt_;
+  std::shared_ptr<TimeParameters<TT>> tparams_;
+  size_t nstages_;
+  std::shared_ptr<Context> ctx_;
+  State U_;
+  Field<double> zb_;
+  State k_;
+  TT t_local_ = 0, dt_ = 0;
+  bool open_ = false;
+  double cn_ = 0.0;
+};
+
+}  // namespace mfsv
