@@ -228,15 +228,15 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc)
   return A;
 }
 
-template <typename T, bool QUICK>
+template <typename T, bool QUICK, bool NANBED>
 void launch_q(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
   dim3 grid((unsigned)c->nblocks), block(kWave * kWavesPerBlock);
   switch (kout ? 0 : nacc) {
-    case 0: hipLaunchKernelGGL((k_stage<T, 0, true, QUICK>), grid, block, 0, c->stream, A); break;
-    case 1: hipLaunchKernelGGL((k_stage<T, 1, false, QUICK>), grid, block, 0, c->stream, A); break;
-    case 2: hipLaunchKernelGGL((k_stage<T, 2, false, QUICK>), grid, block, 0, c->stream, A); break;
-    case 3: hipLaunchKernelGGL((k_stage<T, 3, false, QUICK>), grid, block, 0, c->stream, A); break;
-    default: hipLaunchKernelGGL((k_stage<T, 4, false, QUICK>), grid, block, 0, c->stream, A); break;
+    case 0: hipLaunchKernelGGL((k_stage<T, 0, true, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
+    case 1: hipLaunchKernelGGL((k_stage<T, 1, false, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
+    case 2: hipLaunchKernelGGL((k_stage<T, 2, false, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
+    case 3: hipLaunchKernelGGL((k_stage<T, 3, false, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
+    default: hipLaunchKernelGGL((k_stage<T, 4, false, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
   }
 }
 
@@ -246,13 +246,21 @@ bool pow2(double d) {
   return std::frexp(d, &e) == 0.5;
 }
 
+// Coded-out cells take the exact-quotient path with the reference's NaN
+// rules; otherwise power-of-two spacing (and the FP32 form) takes the
+// difference form, any other spacing exact quotients.
 template <typename T>
 void launch_typed(mfsv_ctx* c, int s, bool kout, bool branches) {
   int nacc = 0;
   const StageArgs<T> A = stage_args<T>(c, s, kout, branches, &nacc);
-  const bool quick = sizeof(T) == 4 || (pow2(c->dx) && pow2(c->dy));
-  if (quick) launch_q<T, true>(c, A, nacc, kout);
-  else launch_q<T, false>(c, A, nacc, kout);
+  if (c->nanbed) {
+    launch_q<T, false, true>(c, A, nacc, kout);
+  } else if constexpr (sizeof(T) == 4) {
+    launch_q<T, true, false>(c, A, nacc, kout);
+  } else {
+    if (pow2(c->dx) && pow2(c->dy)) launch_q<T, true, false>(c, A, nacc, kout);
+    else launch_q<T, false, false>(c, A, nacc, kout);
+  }
 }
 
 int launch_stage(mfsv_ctx* c, int s, bool kout, bool branches) {
@@ -279,7 +287,7 @@ CtrlArgs ctrl_args(const mfsv_ctx* c, int mode) {
 
 int launch_ctrl(mfsv_ctx* c, int mode) {
   const CtrlArgs A = ctrl_args(c, mode);
-  hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(256), 0, c->stream, A);
+  hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kCtrlThreads), 0, c->stream, A);
   HIPCHK(c, hipGetLastError());
   return MFSV_OK;
 }
@@ -371,8 +379,9 @@ int mfsv_device_count(int* count) {
 
 int mfsv_create(const mfsv_grid* g, int precision, int device, mfsv_ctx** out) {
   if (!g || !out || g->nx < 1 || g->ny < 1 || !(g->dx > 0) || !(g->dy > 0)) return MFSV_ERR_ARG;
-  if (g->nx > (1 << 30) || g->ny > (1 << 30) || g->nx * g->ny >= (int64_t(1) << 31))
-    return MFSV_ERR_ARG;  // 32-bit cell indices in the kernels
+  if (g->nx > (1 << 30) || g->ny > (1 << 30) ||
+      g->nx * g->ny * (precision == MFSV_F64 ? 8 : 4) >= (int64_t(1) << 31))
+    return MFSV_ERR_ARG;  // 32-bit byte offsets into each field in the kernels
   if (precision != MFSV_F64 && precision != MFSV_F32) return MFSV_ERR_ARG;
   *out = nullptr;
   mfsv_ctx* c = new (std::nothrow) mfsv_ctx();

@@ -27,6 +27,7 @@ struct mfsv_batch {
   int ntiles = 0;
   bool dirty = true;
   bool quick = true;
+  bool any_nanbed = false;         // a member has coded-out cells
   int nblocks = 0;                 // persistent grid
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -113,8 +114,11 @@ int upload_tables(mfsv_batch* b) {
   bool quick = true;  // exact division stays exact for power-of-two spacing
   if (sizeof(T) == 8)
     for (auto* c : b->active) quick = quick && pow2(c->dx) && pow2(c->dy);
-  if (quick != b->quick) bdrop_graph(b);
+  bool any_nanbed = false;
+  for (auto* c : b->active) any_nanbed = any_nanbed || c->nanbed;
+  if (quick != b->quick || any_nanbed != b->any_nanbed) bdrop_graph(b);
   b->quick = quick;
+  b->any_nanbed = any_nanbed;
   for (int s = 0; s < b->nstages; ++s) {
     std::vector<StageArgs<T>> a(n);
     for (int j = 0; j < n; ++j) {
@@ -145,31 +149,38 @@ int upload_tables(mfsv_batch* b) {
   return MFSV_OK;
 }
 
-template <typename T, bool QUICK>
+template <typename T, bool QUICK, bool ANY_NANBED>
 void launch_batch_stage(mfsv_batch* b, int s) {
   const int* cnt = static_cast<const int*>(b->counts.p);
   const StageArgs<T>* m = static_cast<const StageArgs<T>*>(b->margs[s].p);
   const int2* t = static_cast<const int2*>(b->tiles.p);
   dim3 grid((unsigned)b->nblocks), block(kWave * kWavesPerBlock);
   switch (b->nstages - s) {
-    case 1: hipLaunchKernelGGL((k_stage_batch<T, 1, QUICK>), grid, block, 0, b->stream, m, t, cnt); break;
-    case 2: hipLaunchKernelGGL((k_stage_batch<T, 2, QUICK>), grid, block, 0, b->stream, m, t, cnt); break;
-    case 3: hipLaunchKernelGGL((k_stage_batch<T, 3, QUICK>), grid, block, 0, b->stream, m, t, cnt); break;
-    default: hipLaunchKernelGGL((k_stage_batch<T, 4, QUICK>), grid, block, 0, b->stream, m, t, cnt); break;
+    case 1: hipLaunchKernelGGL((k_stage_batch<T, 1, QUICK, ANY_NANBED>), grid, block, 0, b->stream, m, t, cnt); break;
+    case 2: hipLaunchKernelGGL((k_stage_batch<T, 2, QUICK, ANY_NANBED>), grid, block, 0, b->stream, m, t, cnt); break;
+    case 3: hipLaunchKernelGGL((k_stage_batch<T, 3, QUICK, ANY_NANBED>), grid, block, 0, b->stream, m, t, cnt); break;
+    default: hipLaunchKernelGGL((k_stage_batch<T, 4, QUICK, ANY_NANBED>), grid, block, 0, b->stream, m, t, cnt); break;
   }
+}
+
+template <typename T, bool QUICK>
+void launch_batch_stage_n(mfsv_batch* b, int s) {
+  if (b->any_nanbed) launch_batch_stage<T, QUICK, true>(b, s);
+  else launch_batch_stage<T, QUICK, false>(b, s);
 }
 
 int launch_batch_timestep(mfsv_batch* b) {
   for (int s = 0; s < b->nstages; ++s) {
     if (b->prec == MFSV_F64) {
-      if (b->quick) launch_batch_stage<double, true>(b, s);
-      else launch_batch_stage<double, false>(b, s);
+      if (b->quick) launch_batch_stage_n<double, true>(b, s);
+      else launch_batch_stage_n<double, false>(b, s);
     } else {
-      launch_batch_stage<float, true>(b, s);
+      launch_batch_stage_n<float, true>(b, s);
     }
   }
-  hipLaunchKernelGGL(k_ctrl_batch, dim3((unsigned)std::max(b->cap, 1)), dim3(256), 0, b->stream,
-                     static_cast<const CtrlArgs*>(b->cargs.p), static_cast<const int*>(b->counts.p) + 1);
+  hipLaunchKernelGGL(k_ctrl_batch, dim3((unsigned)std::max(b->cap, 1)), dim3(kCtrlThreads), 0,
+                     b->stream, static_cast<const CtrlArgs*>(b->cargs.p),
+                     static_cast<const int*>(b->counts.p) + 1);
   BHIPCHK(b, hipGetLastError());
   return MFSV_OK;
 }

@@ -17,19 +17,42 @@ struct CtrlArgs {
   int64_t nforcing;
 };
 
-// Fixed-order block sum: thread t sums tiles t, t+256, ... then a tree.
-__device__ __forceinline__ double block_sum(const double* p, int64_t n, double* sh) {
-  double s = 0.0;
-  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) s += p[k];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+constexpr int kCtrlThreads = 1024;
+constexpr int kCtrlWaves = kCtrlThreads / kWave;
+
+// Volume and per-stage source-rate sums of the tile partials, in one pass:
+// thread t accumulates tiles t, t + 1024, ... (independent accumulators),
+// then a butterfly inside each wave and a fixed-order sum over the waves.
+// Deterministic for a given tile count.  out[0] = volume partial sum,
+// out[1 + s] = stage s rate sum.
+__device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double* out) {
+  double acc[1 + kMaxStages];
+#pragma unroll
+  for (int q = 0; q <= kMaxStages; ++q) acc[q] = 0.0;
+  for (int64_t k = threadIdx.x; k < A.ntiles; k += blockDim.x) {
+    acc[0] += A.part_vol[k];
+#pragma unroll
+    for (int s = 0; s < kMaxStages; ++s)
+      if (s < A.nstages) acc[1 + s] += A.part_rate[(int64_t)s * A.ntiles + k];
   }
-  const double r = sh[0];
+#pragma unroll
+  for (int q = 0; q <= kMaxStages; ++q)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, kWave);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & (kWave - 1)) == 0)
+#pragma unroll
+    for (int q = 0; q <= kMaxStages; ++q) sh[w * (1 + kMaxStages) + q] = acc[q];
   __syncthreads();
-  return r;
+  if (threadIdx.x == 0) {
+    const int nw = (int)(blockDim.x >> 6);
+#pragma unroll
+    for (int q = 0; q <= kMaxStages; ++q) {
+      double t = 0.0;
+      for (int i = 0; i < nw; ++i) t += sh[i * (1 + kMaxStages) + q];
+      out[q] = t;
+    }
+  }
 }
 
 // Controller (RungeKutta.cpp:154-167) + accept (RungeKutta.hpp:428-431) +
@@ -37,11 +60,14 @@ __device__ __forceinline__ double block_sum(const double* p, int64_t n, double* 
 __device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
   Ctrl* C = A.ctrl;
   if (C->done) return;
-  const double vol = block_sum(A.part_vol, A.ntiles, sh) * A.area;
-  double rate = 0.0;
-  for (int s = 0; s < A.nstages; ++s)
-    rate += A.b[s] * block_sum(A.part_rate + (int64_t)s * A.ntiles, A.ntiles, sh);
+  double sums[1 + kMaxStages];
+  tile_sums(A, sh, sums);
   if (threadIdx.x != 0) return;
+  const double vol = sums[0] * A.area;
+  double rate = 0.0;
+#pragma unroll
+  for (int s = 0; s < kMaxStages; ++s)
+    if (s < A.nstages) rate += A.b[s] * sums[1 + s];
   const double cn_raw = __longlong_as_double((long long)C->cn_bits);
   C->cn_bits = 0ull;
   C->cn = cn_raw;
@@ -88,16 +114,16 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_ctrl(const CtrlArgs A) {
-  __shared__ double sh[256];
+__global__ __launch_bounds__(kCtrlThreads) void k_ctrl(const CtrlArgs A) {
+  __shared__ double sh[kCtrlWaves * (1 + kMaxStages)];
   ctrl_body(A, sh);
 }
 
 // One block per member of a batch; the member count is read from device
 // memory so that captured graphs survive changes of membership.
-__global__ __launch_bounds__(256) void k_ctrl_batch(const CtrlArgs* __restrict__ args,
-                                                    const int* __restrict__ nmembers_p) {
-  __shared__ double sh[256];
+__global__ __launch_bounds__(kCtrlThreads) void k_ctrl_batch(const CtrlArgs* __restrict__ args,
+                                                             const int* __restrict__ nmembers_p) {
+  __shared__ double sh[kCtrlWaves * (1 + kMaxStages)];
   if ((int)blockIdx.x >= *nmembers_p) return;
   ctrl_body(args[blockIdx.x], sh);
 }

@@ -17,9 +17,11 @@
 // Work decomposition: one wavefront owns a strip of 64 columns (60 interior
 // + 2 halo columns each side) and marches down `rows` rows plus a 2-row halo.
 // The live rows are kept in registers (one row prefetched ahead);
-// x-neighbours move between lanes with cross-lane shuffles; each face and
-// each face difference is evaluated once per wave.  Every row load and store
-// is a contiguous 64-element segment of a structure-of-arrays field.
+// x-neighbours move between lanes with DPP shifts; each face and each face
+// difference is evaluated once per wave.  Every row load and store is a
+// contiguous 64-element segment of a structure-of-arrays field, addressed by
+// a uniform base and a 32-bit offset.  Sources are evaluated branch-free with
+// selects.
 #pragma once
 #include "sv_math.hpp"
 
@@ -98,22 +100,64 @@ struct StageArgs {
 template <typename T> struct Row { T h, u, v, z; };
 template <typename T> struct FaceUV { T h, u, v, z; };
 
-__device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, kWave); }
-__device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWave); }
-__device__ __forceinline__ double shfl_dn1(double v) { return __shfl_down(v, 1, kWave); }
-__device__ __forceinline__ float shfl_dn1(float v) { return __shfl_down(v, 1, kWave); }
+// ---- cross-lane moves: DPP row shifts across the whole wave (wave_shr:1 /
+// wave_shl:1).  from_w(v) in lane l is v of lane l-1, from_e(v) is v of lane
+// l+1; the lane without a source reads 0 (bound_ctrl).  Full-rate VALU moves,
+// no LDS round trip.
+__device__ __forceinline__ float from_w(float v) {
+  return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ double from_w(double v) {
+  return __builtin_amdgcn_update_dpp(0.0, v, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ float from_e(float v) {
+  return __builtin_amdgcn_update_dpp(0.0f, v, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ double from_e(double v) {
+  return __builtin_amdgcn_update_dpp(0.0, v, 0x130, 0xf, 0xf, true);
+}
+template <typename T>
+__device__ __forceinline__ Row<T> from_w(const Row<T>& r) {
+  return {from_w(r.h), from_w(r.u), from_w(r.v), from_w(r.z)};
+}
+template <typename T>
+__device__ __forceinline__ Row<T> from_e(const Row<T>& r) {
+  return {from_e(r.h), from_e(r.u), from_e(r.v), from_e(r.z)};
+}
+template <typename T>
+__device__ __forceinline__ FaceUV<T> from_e(const FaceUV<T>& r) {
+  return {from_e(r.h), from_e(r.u), from_e(r.v), from_e(r.z)};
+}
+// Lane k's value, broadcast (k uniform).
+__device__ __forceinline__ float lane_value(float v, int k) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+}
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// ---- field access: a uniform base pointer plus a 32-bit byte offset per
+// lane (the saddr form of global loads: one shared offset register for all
+// fields of a row).  Lanes and rows outside the grid read a clamped cell,
+// whose value never reaches an owned cell; stores are masked to owned cells.
+template <typename T>
+__device__ __forceinline__ T ld(const T* __restrict__ p, uint32_t off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + off);
+}
+template <typename T>
+__device__ __forceinline__ void st(T* __restrict__ p, uint32_t off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off) = v;
+}
 
 template <typename T>
-__device__ __forceinline__ Row<T> shfl_up_row(const Row<T>& r) {
-  return {shfl_up1(r.h), shfl_up1(r.u), shfl_up1(r.v), shfl_up1(r.z)};
-}
+__device__ __forceinline__ T sel(bool c, T a, T b) { return c ? a : b; }
 template <typename T>
-__device__ __forceinline__ Row<T> shfl_dn_row(const Row<T>& r) {
-  return {shfl_dn1(r.h), shfl_dn1(r.u), shfl_dn1(r.v), shfl_dn1(r.z)};
-}
-template <typename T>
-__device__ __forceinline__ FaceUV<T> shfl_dn_face(const FaceUV<T>& r) {
-  return {shfl_dn1(r.h), shfl_dn1(r.u), shfl_dn1(r.v), shfl_dn1(r.z)};
+__device__ __forceinline__ Side<T> sel(bool c, const Side<T>& a, const Side<T>& b) {
+  return {c ? a.h : b.h, c ? a.un : b.un, c ? a.ut : b.ut, c ? a.z : b.z,
+          c ? a.sh : b.sh, c ? a.sun : b.sun, c ? a.sut : b.sut, c ? a.sz : b.sz};
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that runs
@@ -143,12 +187,34 @@ __device__ __forceinline__ T storm_axis(const Storm& s, double a, double c) {
 __device__ __forceinline__ double vdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float vdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// Division by the (uniform) timestep: a / dt correctly rounded in FP64 from
+// rdt = RN(1/dt) (div_c), reciprocal multiply in the FP32 form.
+__device__ __forceinline__ double div_dt(double a, double dt, double rdt) { return div_c(a, dt, rdt); }
+__device__ __forceinline__ float div_dt(float a, float, float rdt) { return a * rdt; }
+
+// E2 ghost of the x = 0 edge face: zero-gradient depth and tangential
+// velocity, the normal velocity only where it drains out of the domain.
+template <typename T>
+__device__ __forceinline__ Side<T> outflow_ghost(const Side<T>& c) {
+  Side<T> g;
+  g.h = c.h; g.un = fmin_t(c.un, T(0)); g.ut = c.ut; g.z = c.z;
+  g.sh = T(0); g.sun = T(0); g.sut = T(0); g.sz = T(0);
+  return g;
+}
+
 // NACC: accumulator rows this stage writes (0 with KOUT: write k instead).
-// QUICK: divide by dx, dy with one multiplication by the reciprocal (exact
-// for power-of-two spacings; the FP32 form always does).
-template <typename T, int NACC, bool KOUT, bool QUICK>
+// QUICK: power-of-two spacing (always in the FP32 form).  Slopes are then
+// carried as per-cell differences m = d * slope: for d = 2^k the Minmod3 of
+// the quotients is exactly the Minmod3 of the differences scaled by 1/d, the
+// face projection (d/2) * slope is exactly m/2, and -g * slope is exactly
+// m * (-g/d), so every value is the reference's bit for bit.  Otherwise
+// (and with coded-out cells, NANBED) slopes are exact quotients.
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED>
 __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile, const int lane) {
+  static_assert(!(QUICK && NANBED), "coded-out cells take the quotient path");
   constexpr bool kRel = sizeof(T) == 4;
+  constexpr uint32_t S = sizeof(T);
+  constexpr int NB = NACC > 0 ? NACC : 1;
   Ctrl* __restrict__ C = A.ctrl;
   if (C->done) return;
 
@@ -161,41 +227,63 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   const bool own = lane >= kHalo && lane < kWave - kHalo && xin;
   const bool hasW = x > 0, hasE = x < nx - 1;
   const bool e7 = (A.flags & 0x40) != 0;
-  const bool nanbed = A.nanbed != 0;
+  const bool e2 = (A.flags & 0x02) != 0;
+  // does this strip hold the x = 0 or the x = nx face?
+  const bool edge_strip = strip == 0 || (strip * kStrip - kHalo <= nx && nx < strip * kStrip - kHalo + kWave);
 
   const double dt = C->dt;
   const double tsub = C->t_local + A.c_s * dt;
   const int cur = C->cur;
   // select, not index: a run-time index into the argument block would copy
   // it to private memory
-  T* const Uh0 = cur ? A.Uh[1] : A.Uh[0], * const Uh1 = cur ? A.Uh[0] : A.Uh[1];
-  T* const Uu0 = cur ? A.Uu[1] : A.Uu[0], * const Uu1 = cur ? A.Uu[0] : A.Uu[1];
-  T* const Uv0 = cur ? A.Uv[1] : A.Uv[0], * const Uv1 = cur ? A.Uv[0] : A.Uv[1];
-  const T* __restrict__ xh = A.in_h ? A.in_h : Uh0;
-  const T* __restrict__ xu = A.in_u ? A.in_u : Uu0;
-  const T* __restrict__ xv = A.in_v ? A.in_v : Uv0;
+  T* const Uh1 = cur ? A.Uh[0] : A.Uh[1];
+  T* const Uu1 = cur ? A.Uu[0] : A.Uu[1];
+  T* const Uv1 = cur ? A.Uv[0] : A.Uv[1];
+  const T* const xh = A.in_h ? A.in_h : (cur ? A.Uh[1] : A.Uh[0]);
+  const T* const xu = A.in_u ? A.in_u : (cur ? A.Uu[1] : A.Uu[0]);
+  const T* const xv = A.in_v ? A.in_v : (cur ? A.Uv[1] : A.Uv[0]);
   const T* __restrict__ zb = A.zb;
-  const T dx = A.dx, dy = A.dy, rdx = A.rdx, rdy = A.rdy, hdx = A.hdx, hdy = A.hdy;
+  const T* __restrict__ nf = A.nfield;
+  const bool has_n = nf != nullptr;
+  const T dx = A.dx, dy = A.dy, rdx = A.rdx, rdy = A.rdy;
+  const T hdx = QUICK ? T(0.5) : A.hdx, hdy = QUICK ? T(0.5) : A.hdy;
+  const T gx = QUICK ? T(-kG) * rdx : T(-kG), gy = QUICK ? T(-kG) * rdy : T(-kG);
   const T G = T(kG);
 
   // RK coefficients of this stage: (T)(a * dt) as State::operator*(ValueType).
-  T sc[NACC > 0 ? NACC : 1];
-  T* dh_[NACC > 0 ? NACC : 1];
-  T* du_[NACC > 0 ? NACC : 1];
-  T* dv_[NACC > 0 ? NACC : 1];
+  T sc[NB];
+  T* oh_[NB];
+  T* ou_[NB];
+  T* ov_[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    sc[j] = T(0);
+    oh_[j] = Uh1; ou_[j] = Uu1; ov_[j] = Uv1;
+  }
 #pragma unroll
   for (int j = 0; j < NACC; ++j) {
     sc[j] = T(A.acoef[j] * dt);
-    dh_[j] = A.acc_h[j] ? A.acc_h[j] : Uh1;
-    du_[j] = A.acc_u[j] ? A.acc_u[j] : Uu1;
-    dv_[j] = A.acc_v[j] ? A.acc_v[j] : Uv1;
+    oh_[j] = A.acc_h[j] ? A.acc_h[j] : Uh1;
+    ou_[j] = A.acc_u[j] ? A.acc_u[j] : Uu1;
+    ov_[j] = A.acc_v[j] ? A.acc_v[j] : Uv1;
   }
+
+  // byte offsets of row r in this lane's column (clamped to the grid)
+  const int xc_ = x < 0 ? 0 : (x < nx ? x : nx - 1);
+  const uint32_t lane_off = uint32_t(xc_) * S;
+  const uint32_t row_bytes = uint32_t(nx) * S;
+  auto roff = [&](int r) -> uint32_t {
+    const int rc = r < 0 ? 0 : (r < ny ? r : ny - 1);
+    return lane_off + uint32_t(rc) * row_bytes;
+  };
 
   // Uniform discharge / head terms (global selection, scalar endpoints).
   const double step_len = C->step_dur;
   const double q0s = C->q0, q1s = C->q1, hb0s = C->hb0, hb1s = C->hb1;
   const double gstep = C->gstep;
   const bool storm = (A.flags & 0x08) != 0;
+  const T dtT = T(dt);
+  const T rdt = sizeof(T) == 8 ? T(1.0 / dt) : T(1.0f / float(dt));
   // E4 storm factors: this lane's column at the step's two endpoint times,
   // and (in lane k) row y0 + k, broadcast to all lanes row by row.
   T ex0 = T(0), ex1 = T(0), eyA0 = T(0), eyA1 = T(0);
@@ -211,20 +299,22 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   }
 
   auto load = [&](int r) {
-    Row<T> R;
-    if (xin && r >= 0 && r < ny) {
-      const int i = r * nx + x;
-      R.h = xh[i]; R.u = xu[i]; R.v = xv[i]; R.z = zb[i];
-    } else {
-      R.h = T(0); R.u = T(0); R.v = T(0); R.z = T(0);
-    }
-    return R;
+    const uint32_t o = roff(r);
+    return Row<T>{ld(xh, o), ld(xu, o), ld(xv, o), ld(zb, o)};
+  };
+  // a difference along x / y as the slope quantity (see QUICK)
+  auto qx = [&](T d) -> T {
+    if constexpr (QUICK) return d;
+    else return div_c(d, dx, rdx);
+  };
+  auto qy = [&](T d) -> T {
+    if constexpr (QUICK) return d;
+    else return div_c(d, dy, rdy);
   };
 
   Row<T> R0 = load(y0 - 2), R1 = load(y0 - 1), R2 = load(y0);
   // Per-cell inputs of the cell pass (Manning n, accumulator bases of a
   // later stage), loaded one row before the row that uses them.
-  constexpr int NB = NACC > 0 ? NACC : 1;
   struct Extra {
     T n;
     T bh[NB], bu[NB], bv[NB];
@@ -232,161 +322,169 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   const bool base_ld = A.stage > 0;
   auto load_extra = [&](int r) {
     Extra e;
-    e.n = A.n_sh;
+    const uint32_t o = roff(r);
+    e.n = has_n ? ld(nf, o) : A.n_sh;
 #pragma unroll
     for (int j = 0; j < NB; ++j) { e.bh[j] = T(0); e.bu[j] = T(0); e.bv[j] = T(0); }
-    if (own && r >= y0 && r < y1) {
-      const int i = r * nx + x;
-      if (A.nfield) e.n = A.nfield[i];
-      if (base_ld) {
+    if (base_ld) {
 #pragma unroll
-        for (int j = 0; j < NACC; ++j) { e.bh[j] = dh_[j][i]; e.bu[j] = du_[j][i]; e.bv[j] = dv_[j][i]; }
+      for (int j = 0; j < NACC; ++j) {
+        e.bh[j] = ld(oh_[j], o); e.bu[j] = ld(ou_[j], o); e.bv[j] = ld(ov_[j], o);
       }
     }
     return e;
   };
   Extra X0 = load_extra(y0 - 1);
-  // south quotients (U_c - U_{c-1})/dy of the current row, carried down
-  // (first row: rows y0-2 and y0-1; unused when y0-1 has no south neighbour)
-  T qsh = dv<QUICK>(R1.h - R0.h, dy, rdy), qsu = dv<QUICK>(R1.u - R0.u, dy, rdy);
-  T qsv = dv<QUICK>(R1.v - R0.v, dy, rdy), qsz = dv<QUICK>(R1.z - R0.z, dy, rdy);
+  // south slope quantities of the current row (rows y0-2 and y0-1 first)
+  T qsh = qy(R1.h - R0.h), qsu = qy(R1.u - R0.u), qsv = qy(R1.v - R0.v), qsz = qy(R1.z - R0.z);
   // slopes of row c-1 that the cell pass and the y-face still need
   T p_hx = T(0), p_zx = T(0), p_hy = T(0), p_uy = T(0), p_vy = T(0), p_zy = T(0);
   FaceUV<T> Fs0 = {}, Xw0 = {};
-  double vol = 0.0, rate = 0.0, cnmax = 0.0;
+  double vol = 0.0, rate = 0.0;
+  T qmax = T(0);
 
-#pragma unroll 3
+#pragma unroll 2
   for (int c = y0 - 1; c <= y1; ++c) {
     const Row<T> Rn = load(c + 2);  // prefetch
     const Extra X1 = load_extra(c);
-    const bool rowin = c >= 0 && c < ny;
-    const Row<T> W = shfl_up_row(R1), E = shfl_dn_row(R1);
+    const Row<T> W = from_w(R1), E = from_e(R1);
+    const bool hasN = c + 1 < ny, hasS = c > 0;
     // ---- slopes of row c (State.cpp:187-192, SpatialDerivative.hpp:104-122)
-    T shx = T(0), sux = T(0), svx = T(0), szx = T(0);
-    T shy = T(0), suy = T(0), svy = T(0), szy = T(0);
+    T shx, sux, svx, szx, shy, suy, svy, szy;
     T qnh = T(0), qnu = T(0), qnv = T(0), qnz = T(0);
-    if (rowin) {
-      const bool hasN = c + 1 < ny, hasS = c > 0;
-      if (!nanbed) {
-        const T qwh = dv<QUICK>(R1.h - W.h, dx, rdx), qwu = dv<QUICK>(R1.u - W.u, dx, rdx);
-        const T qwv = dv<QUICK>(R1.v - W.v, dx, rdx), qwz = dv<QUICK>(R1.z - W.z, dx, rdx);
-        const T qeh = shfl_dn1(qwh), qeu = shfl_dn1(qwu), qev = shfl_dn1(qwv), qez = shfl_dn1(qwz);
-        shx = minmod_q(qwh, qeh, dv<QUICK>(E.h - W.h, dx, rdx), hasW, hasE);
-        sux = minmod_q(qwu, qeu, dv<QUICK>(E.u - W.u, dx, rdx), hasW, hasE);
-        svx = minmod_q(qwv, qev, dv<QUICK>(E.v - W.v, dx, rdx), hasW, hasE);
-        szx = minmod_q(qwz, qez, dv<QUICK>(E.z - W.z, dx, rdx), hasW, hasE);
-        qnh = dv<QUICK>(R2.h - R1.h, dy, rdy); qnu = dv<QUICK>(R2.u - R1.u, dy, rdy);
-        qnv = dv<QUICK>(R2.v - R1.v, dy, rdy); qnz = dv<QUICK>(R2.z - R1.z, dy, rdy);
-        // south -> north slope; the reference's north -> south slope is its
-        // exact negation (Minmod3 commutes with negation)
-        const T yh = minmod_q(qsh, qnh, dv<QUICK>(R2.h - R0.h, dy, rdy), hasS, hasN);
-        const T yu = minmod_q(qsu, qnu, dv<QUICK>(R2.u - R0.u, dy, rdy), hasS, hasN);
-        const T yv = minmod_q(qsv, qnv, dv<QUICK>(R2.v - R0.v, dy, rdy), hasS, hasN);
-        const T yz = minmod_q(qsz, qnz, dv<QUICK>(R2.z - R0.z, dy, rdy), hasS, hasN);
-        shy = e7 ? yh : -yh; suy = e7 ? yu : -yu; svy = e7 ? yv : -yv; szy = e7 ? yz : -yz;
+    if constexpr (!NANBED) {
+      const T qwh = qx(R1.h - W.h), qwu = qx(R1.u - W.u), qwv = qx(R1.v - W.v), qwz = qx(R1.z - W.z);
+      const T qeh = from_e(qwh), qeu = from_e(qwu), qev = from_e(qwv), qez = from_e(qwz);
+      shx = minmod_q(qwh, qeh, qx(E.h - W.h), hasW, hasE);
+      sux = minmod_q(qwu, qeu, qx(E.u - W.u), hasW, hasE);
+      svx = minmod_q(qwv, qev, qx(E.v - W.v), hasW, hasE);
+      szx = minmod_q(qwz, qez, qx(E.z - W.z), hasW, hasE);
+      qnh = qy(R2.h - R1.h); qnu = qy(R2.u - R1.u); qnv = qy(R2.v - R1.v); qnz = qy(R2.z - R1.z);
+      // south -> north slope; the reference's north -> south slope is its
+      // exact negation (Minmod3 commutes with negation)
+      const T yh = minmod_q(qsh, qnh, qy(R2.h - R0.h), hasS, hasN);
+      const T yu = minmod_q(qsu, qnu, qy(R2.u - R0.u), hasS, hasN);
+      const T yv = minmod_q(qsv, qnv, qy(R2.v - R0.v), hasS, hasN);
+      const T yz = minmod_q(qsz, qnz, qy(R2.z - R0.z), hasS, hasN);
+      shy = e7 ? yh : -yh; suy = e7 ? yu : -yu; svy = e7 ? yv : -yv; szy = e7 ? yz : -yz;
+    } else {
+      // general Minmod3 with the reference's NaN rules
+      const Row<T> Wc = hasW ? W : R1, Ec = hasE ? E : R1;
+      shx = minmod3(Wc.h, hasW, R1.h, hasE, Ec.h, dx, rdx);
+      sux = minmod3(Wc.u, hasW, R1.u, hasE, Ec.u, dx, rdx);
+      svx = minmod3(Wc.v, hasW, R1.v, hasE, Ec.v, dx, rdx);
+      szx = minmod3(Wc.z, hasW, R1.z, hasE, Ec.z, dx, rdx);
+      const Row<T> N = hasN ? R2 : R1, Sr = hasS ? R0 : R1;
+      if (e7) {
+        shy = minmod3(Sr.h, hasS, R1.h, hasN, N.h, dy, rdy);
+        suy = minmod3(Sr.u, hasS, R1.u, hasN, N.u, dy, rdy);
+        svy = minmod3(Sr.v, hasS, R1.v, hasN, N.v, dy, rdy);
+        szy = minmod3(Sr.z, hasS, R1.z, hasN, N.z, dy, rdy);
       } else {
-        // general Minmod3 with the reference's NaN rules
-        const Row<T> Wc = hasW ? W : R1, Ec = hasE ? E : R1;
-        shx = minmod3(Wc.h, hasW, R1.h, hasE, Ec.h, dx, rdx);
-        sux = minmod3(Wc.u, hasW, R1.u, hasE, Ec.u, dx, rdx);
-        svx = minmod3(Wc.v, hasW, R1.v, hasE, Ec.v, dx, rdx);
-        szx = minmod3(Wc.z, hasW, R1.z, hasE, Ec.z, dx, rdx);
-        const Row<T> N = hasN ? R2 : R1, S = hasS ? R0 : R1;
-        if (e7) {
-          shy = minmod3(S.h, hasS, R1.h, hasN, N.h, dy, rdy);
-          suy = minmod3(S.u, hasS, R1.u, hasN, N.u, dy, rdy);
-          svy = minmod3(S.v, hasS, R1.v, hasN, N.v, dy, rdy);
-          szy = minmod3(S.z, hasS, R1.z, hasN, N.z, dy, rdy);
-        } else {
-          shy = minmod3(N.h, hasN, R1.h, hasS, S.h, dy, rdy);
-          suy = minmod3(N.u, hasN, R1.u, hasS, S.u, dy, rdy);
-          svy = minmod3(N.v, hasN, R1.v, hasS, S.v, dy, rdy);
-          szy = minmod3(N.z, hasN, R1.z, hasS, S.z, dy, rdy);
-        }
+        shy = minmod3(N.h, hasN, R1.h, hasS, Sr.h, dy, rdy);
+        suy = minmod3(N.u, hasN, R1.u, hasS, Sr.u, dy, rdy);
+        svy = minmod3(N.v, hasN, R1.v, hasS, Sr.v, dy, rdy);
+        szy = minmod3(N.z, hasN, R1.z, hasS, Sr.z, dy, rdy);
       }
     }
     // ---- west x-face of row c (face id c*(nx+1) + x)
-    const T Whx = shfl_up1(shx), Wux = shfl_up1(sux), Wvx = shfl_up1(svx), Wzx = shfl_up1(szx);
     FaceUV<T> Xw1 = {};
     if (c >= y0 && c < y1) {
-      Side<T> R = {R1.h, R1.u, R1.v, R1.z, shx, sux, svx, szx};
-      Side<T> L = {W.h, W.u, W.v, W.z, Whx, Wux, Wvx, Wzx};
+      const Side<T> R = {R1.h, R1.u, R1.v, R1.z, shx, sux, svx, szx};
+      Side<T> L = {W.h, W.u, W.v, W.z, from_w(shx), from_w(sux), from_w(svx), from_w(szx)};
       float br;
       Flux<T> F;
-      if (x == 0 && (A.flags & 0x02)) {  // E2: zero-gradient ghost that only drains
-        L = {R1.h, fmin_t(R1.u, T(0)), R1.v, R1.z, T(0), T(0), T(0), T(0)};
-        F = face_flux<T, kRel>(L, R, 0, hdx, dx, rdx, &br);
-      } else if (!nanbed) {
-        if (x == 0) L = wall_ghost(R);
-        else if (x == nx) R = wall_ghost(L);
-        F = face_fast<T, kRel, QUICK>(L, R, hdx, dx, rdx, &br);
+      if constexpr (!NANBED) {
+        Side<T> Rf = R;
+        if (edge_strip) {
+          L = sel(x == 0, e2 ? outflow_ghost(R) : wall_ghost(R), L);
+          Rf = sel(x == nx, wall_ghost(L), R);
+        }
+        F = face_fast<T, kRel, QUICK>(L, Rf, hdx, dx, rdx, &br);
       } else {
-        int edge = 0;
-        if (x == 0) { L = R; edge = -1; }
-        else if (x == nx) { R = L; edge = 1; }
-        F = face_flux<T, kRel>(L, R, edge, hdx, dx, rdx, &br);
+        if (x == 0 && e2) {
+          F = face_flux<T, kRel>(outflow_ghost(R), R, 0, hdx, dx, rdx, &br);
+        } else {
+          Side<T> Rf = R;
+          int edge = 0;
+          if (x == 0) { L = R; edge = -1; }
+          else if (x == nx) { Rf = L; edge = 1; }
+          F = face_flux<T, kRel>(L, Rf, edge, hdx, dx, rdx, &br);
+        }
       }
       Xw1 = {F.h, F.n, F.t, F.z};
-      if (A.br_x && x >= 0 && x <= nx && lane >= kHalo && lane <= kWave - kHalo)
-        A.br_x[c * (nx + 1) + x] = br;
+      if constexpr (KOUT) {
+        if (A.br_x && x >= 0 && x <= nx && lane >= kHalo && lane <= kWave - kHalo)
+          A.br_x[c * (nx + 1) + x] = br;
+      }
     }
     // ---- south y-face of row c (face id (nx+1)ny + c*nx + x)
     FaceUV<T> Fs1 = {};
     if (c >= y0) {
-      Side<T> L, R;
       float br;
       Flux<T> F;
       if (c == ny) {  // north edge of the top row
-        L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
-        if (!nanbed) F = face_fast<T, kRel, QUICK>(L, wall_ghost(L), hdy, dy, rdy, &br);
+        const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
+        if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(L, wall_ghost(L), hdy, dy, rdy, &br);
         else F = face_flux<T, kRel>(L, L, 1, hdy, dy, rdy, &br);
       } else {
-        R = {R1.h, R1.v, R1.u, R1.z, shy, svy, suy, szy};
+        const Side<T> R = {R1.h, R1.v, R1.u, R1.z, shy, svy, suy, szy};
         if (c == 0) {
-          if (!nanbed) F = face_fast<T, kRel, QUICK>(wall_ghost(R), R, hdy, dy, rdy, &br);
+          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(wall_ghost(R), R, hdy, dy, rdy, &br);
           else F = face_flux<T, kRel>(R, R, -1, hdy, dy, rdy, &br);
         } else {
-          L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
-          if (!nanbed) F = face_fast<T, kRel, QUICK>(L, R, hdy, dy, rdy, &br);
+          const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
+          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(L, R, hdy, dy, rdy, &br);
           else F = face_flux<T, kRel>(L, R, 0, hdy, dy, rdy, &br);
         }
       }
       Fs1 = {F.h, F.t, F.n, F.z};
-      if (A.br_y && own) A.br_y[c * nx + x] = br;
-    }
-    // ---- cell pass + sources + RK for row r = c - 1
-    const FaceUV<T> Xe0 = shfl_dn_face(Xw0);
-    const int r = c - 1;
-    T ey0 = T(0), ey1 = T(0);
-    if (storm) {  // uniform: every lane takes part in the broadcast
-      const int k = r - y0;
-      if (k >= 0 && k < kWave) {
-        ey0 = __shfl(eyA0, k, kWave);
-        ey1 = __shfl(eyA1, k, kWave);
-      } else {
-        const Storm& st = A.storm;
-        const double yc = A.oy + (double(r) + 0.5) * A.dyd;
-        ey0 = storm_axis<T>(st, yc, st.cy0 + st.vy * gstep);
-        ey1 = storm_axis<T>(st, yc, st.cy0 + st.vy * (gstep + step_len));
+      if constexpr (KOUT) {
+        if (A.br_y && own) A.br_y[c * nx + x] = br;
       }
     }
-    if (r >= y0 && r < y1 && own) {
-      const int i = r * nx + x;
+    // ---- cell pass + sources + RK for row r = c - 1
+    const FaceUV<T> Xe0 = from_e(Xw0);
+    const int r = c - 1;
+    if (r >= y0 && r < y1) {
+      T ey0 = T(0), ey1 = T(0);
+      if (storm) {
+        const int k = r - y0;
+        if (k < kWave) {
+          ey0 = lane_value(eyA0, k);
+          ey1 = lane_value(eyA1, k);
+        } else {
+          const Storm& st = A.storm;
+          const double yc = A.oy + (double(r) + 0.5) * A.dyd;
+          ey0 = storm_axis<T>(st, yc, st.cy0 + st.vy * gstep);
+          ey1 = storm_axis<T>(st, yc, st.cy0 + st.vy * (gstep + step_len));
+        }
+      }
+      const uint32_t wo = lane_off + uint32_t(r) * row_bytes;
       const T h = R0.h, u = R0.u, v = R0.v;
       // TemporalDerivativeKernel.cpp:57-90
       T dhdt = dv<QUICK>(Xw0.h - Xe0.h, dx, rdx) + dv<QUICK>(Fs0.h - Fs1.h, dy, rdy);
       T dudt = dv<QUICK>(Xw0.u - Xe0.u, dx, rdx) + dv<QUICK>(Fs0.u - Fs1.u, dy, rdy);
       T dvdt = dv<QUICK>(Xw0.v - Xe0.v, dx, rdx) + dv<QUICK>(Fs0.v - Fs1.v, dy, rdy);
-      dudt += p_hx * T(-kG);
-      dvdt += p_hy * T(-kG);
-      T dzdx = p_zx;
-      const T hx_ = dv<QUICK>(h, dx, rdx);
-      if (fabs_t(dzdx) > hx_) dzdx = sy_sign(dzdx) * hx_;
-      T dzdy = p_zy;
-      const T hy_ = dv<QUICK>(h, dy, rdy);
-      if (fabs_t(dzdy) > hy_) dzdy = sy_sign(dzdy) * hy_;
-      const T dudt_bed = T(-kG) * dzdx;
-      const T dvdt_bed = T(-kG) * dzdy;
+      dudt += p_hx * gx;
+      dvdt += p_hy * gy;
+      // bed slope limited to |dz/dx| <= h/dx
+      T dudt_bed, dvdt_bed;
+      if constexpr (QUICK) {
+        const T zx = (fabs_t(p_zx) > h) ? sy_sign(p_zx) * h : p_zx;
+        const T zy = (fabs_t(p_zy) > h) ? sy_sign(p_zy) * h : p_zy;
+        dudt_bed = gx * zx;
+        dvdt_bed = gy * zy;
+      } else {
+        T dzdx = p_zx;
+        const T hx_ = div_c(h, dx, rdx);
+        if (fabs_t(dzdx) > hx_) dzdx = sy_sign(dzdx) * hx_;
+        T dzdy = p_zy;
+        const T hy_ = div_c(h, dy, rdy);
+        if (fabs_t(dzdy) > hy_) dzdy = sy_sign(dzdy) * hy_;
+        dudt_bed = T(-kG) * dzdx;
+        dvdt_bed = T(-kG) * dzdy;
+      }
       dudt += dv<QUICK>(Xw0.z - Xe0.z, dx, rdx);
       dvdt += dv<QUICK>(Fs0.z - Fs1.z, dy, rdy);
       dudt += dudt_bed;
@@ -394,71 +492,82 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       // ManningRoughnessSourceTerm.cpp:55-90
       if (A.manning) {
         T nsh = A.n_sh, ndp = A.n_dp;
-        if (A.nfield) { nsh = X0.n; ndp = nsh; }
+        if (has_n) { nsh = X0.n; ndp = nsh; }
         T n = nsh;  // mix(a, a, t) == a
         if (nsh != ndp) {
           T t = vdiv(h - A.d_sh, A.d_dp - A.d_sh);
           t = fmin_t(fmax_t(t, T(0)), T(1));
           n = nsh + (ndp - nsh) * (t * t * (T(3) - T(2) * t));
         }
-        if (h > T(1e-6)) {
-          const T inv_h = vdiv(h, h * h + T(1e-3));
-          const T Sf = n * n * fsqrt_t(u * u + v * v) * pow43(inv_h);
-          T dum = T(-kG) * Sf * u;
-          T dvm = T(-kG) * Sf * v;
-          const double ts = dt;
-          if (fabs_t(T(dum * ts)) > fabs_t(u) && sy_sign(T(dum * ts)) != sy_sign(u)) dum = T(-u / ts);
-          if (fabs_t(T(dvm * ts)) > fabs_t(v) && sy_sign(T(dvm * ts)) != sy_sign(v)) dvm = T(-v / ts);
-          dudt += dum;
-          dvdt += dvm;
-        }
+        const bool wet = h > T(1e-6);
+        const T inv_h = vdiv(h, h * h + T(1e-3));
+        const T Sf = n * n * fsqrt_t(u * u + v * v) * pow43(inv_h);
+        T dum = T(-kG) * Sf * u;
+        T dvm = T(-kG) * Sf * v;
+        // overshoot cap with the full timestep
+        const T pu = dum * dtT, pv = dvm * dtT;
+        if (fabs_t(pu) > fabs_t(u) && sy_sign(pu) != sy_sign(u)) dum = div_dt(-u, dtT, rdt);
+        if (fabs_t(pv) > fabs_t(v) && sy_sign(pv) != sy_sign(v)) dvm = div_dt(-v, dtT, rdt);
+        dudt = wet ? dudt + dum : dudt;
+        dvdt = wet ? dvdt + dvm : dvdt;
       }
       // DischargeBoundarySourceTerm.cpp:43-67 (E1 flips the guard)
       if (A.discharge || storm) {
         T tq0, tq1;
-        if (A.discharge == 2) { tq0 = A.q0f[i]; tq1 = A.q1f[i]; }
+        if (A.discharge == 2) { tq0 = ld(A.q0f, wo); tq1 = ld(A.q1f, wo); }
         else if (A.discharge == 1) { tq0 = T(q0s); tq1 = T(q1s); }
         else { tq0 = T(0); tq1 = T(0); }
         if (storm) {
           tq0 += T(A.storm.peak) * ex0 * ey0;
           tq1 += T(A.storm.peak) * ex1 * ey1;
         }
-        if (tq0 != T(0) || tq1 != T(0)) {
-          T dh;
-          if constexpr (sizeof(T) == 8) {
-            const double dqdt = div_c(tq1 - tq0, step_len, A.rstep);
-            const double qnow = tq0 + tsub * dqdt;
-            const double qnext = qnow + dt * dqdt;
-            dh = 0.5 * (qnow + qnext);
-          } else {
-            const float dqdt = (tq1 - tq0) * float(A.rstep);
-            const float qnow = __builtin_fmaf(float(tsub), dqdt, tq0);
-            const float qnext = __builtin_fmaf(float(dt), dqdt, qnow);
-            dh = 0.5f * (qnow + qnext);
-          }
-          const bool dry = (A.flags & 0x01) ? (h + dh * dt <= 0.0) : (h - dh * dt <= 0.0);
-          if (dry) dh = T(-h / dt);
-          dhdt += dh;
+        T dh;
+        if constexpr (sizeof(T) == 8) {
+          const double dqdt = div_c(tq1 - tq0, step_len, A.rstep);
+          const double qnow = tq0 + tsub * dqdt;
+          const double qnext = qnow + dt * dqdt;
+          dh = 0.5 * (qnow + qnext);
+        } else {
+          const float dqdt = (tq1 - tq0) * float(A.rstep);
+          const float qnow = __builtin_fmaf(float(tsub), dqdt, tq0);
+          const float qnext = __builtin_fmaf(dtT, dqdt, qnow);
+          dh = 0.5f * (qnow + qnext);
         }
+        const bool dry = (A.flags & 0x01) ? (h + dh * dtT <= T(0)) : (h - dh * dtT <= T(0));
+        dh = dry ? div_dt(-h, dtT, rdt) : dh;
+        dhdt = (tq0 != T(0) || tq1 != T(0)) ? dhdt + dh : dhdt;
       }
       // HeadBoundarySourceTerm.cpp:43-66 (overwrites dh)
       if (A.head) {
         T h0, h1;
-        if (A.head == 2) { h0 = A.hb0f[i]; h1 = A.hb1f[i]; }
-        else if (x >= A.hx0 && x < A.hx1) { h0 = T(hb0s); h1 = T(hb1s); }
-        else { h0 = Num<T>::nan(); h1 = h0; }
-        if (!(h0 != h0) && !(h1 != h1)) {
-          const T dhb = T((h1 - h0) / step_len);
-          const T now = T(h0 + tsub * dhb);
-          const T next = T(now + dt * dhb);
-          T target = T(0.5) * (now + next);
-          if (target <= T(0)) target = T(0);
-          dhdt = T((target - h) / dt);
+        if (A.head == 2) { h0 = ld(A.hb0f, wo); h1 = ld(A.hb1f, wo); }
+        else {
+          const bool in = x >= A.hx0 && x < A.hx1;
+          h0 = in ? T(hb0s) : Num<T>::nan();
+          h1 = in ? T(hb1s) : Num<T>::nan();
         }
+        T target;
+        if constexpr (sizeof(T) == 8) {
+          const double dhb = div_c(h1 - h0, step_len, A.rstep);
+          const double now = h0 + tsub * dhb;
+          const double next = now + dt * dhb;
+          target = 0.5 * (now + next);
+        } else {
+          const float dhb = (h1 - h0) * float(A.rstep);
+          const float now = __builtin_fmaf(float(tsub), dhb, h0);
+          const float next = __builtin_fmaf(dtT, dhb, now);
+          target = 0.5f * (now + next);
+        }
+        target = (target <= T(0)) ? T(0) : target;
+        dhdt = (!(h0 != h0) && !(h1 != h1)) ? div_dt(target - h, dtT, rdt) : dhdt;
       }
-      rate += double(dhdt);
-      if (KOUT) {
-        A.k_h[i] = dhdt; A.k_u[i] = dudt; A.k_v[i] = dvdt;
+      rate += own ? double(dhdt) : 0.0;
+      if constexpr (KOUT) {
+        if (own) {
+          st(A.k_h, wo, dhdt);
+          st(A.k_u, wo, dudt);
+          st(A.k_v, wo, dvdt);
+        }
       } else {
         // RungeKutta.hpp:377-388: y_i = y_i + k_s * (a dt)
         T oh = h, ou = u, ov = v;
@@ -470,22 +579,29 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           T nu = bu + dudt * sc[j];
           T nv = bv + dvdt * sc[j];
           if (j == 0) {  // row s+1 is complete after this stage
-            if ((A.flags & 0x80) && nh <= A.h_dry) { nu = T(0); nv = T(0); }  // E8
-            if (A.flags & 0x100) {                                           // E9
+            if (A.flags & 0x80) {  // E8
+              const bool dryc = nh <= A.h_dry;
+              nu = dryc ? T(0) : nu;
+              nv = dryc ? T(0) : nv;
+            }
+            if (A.flags & 0x100) {  // E9
               nu = fmin_t(fmax_t(nu, -A.u_max), A.u_max);
               nv = fmin_t(fmax_t(nv, -A.u_max), A.u_max);
             }
             oh = nh; ou = nu; ov = nv;
           }
-          dh_[j][i] = nh; du_[j][i] = nu; dv_[j][i] = nv;
+          if (own) {
+            st(oh_[j], wo, nh);
+            st(ou_[j], wo, nu);
+            st(ov_[j], wo, nv);
+          }
         }
         if (A.last) {
-          // State.cpp:211-222 on the new state
-          const T hh = fmax_t(oh, T(0));
-          const T cc = fsqrt_t(G * hh);
-          const double cn = dt * double(dv<QUICK>(fabs_t(ou) + cc, dx, rdx) + dv<QUICK>(fabs_t(ov) + cc, dy, rdy));
-          if (cn > cnmax) cnmax = cn;
-          vol += double(oh);
+          // State.cpp:211-222 on the new state; max_i dt*q_i == dt*max_i q_i
+          const T cc = fsqrt_t(G * fmax_t(oh, T(0)));
+          const T q = dv<QUICK>(fabs_t(ou) + cc, dx, rdx) + dv<QUICK>(fabs_t(ov) + cc, dy, rdy);
+          qmax = (own && q > qmax) ? q : qmax;
+          vol += own ? double(oh) : 0.0;
         }
       }
     }
@@ -498,6 +614,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   }
 
   // wave reductions (fixed butterfly order -> deterministic per tile)
+  double cnmax = dt * double(qmax);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     rate += __shfl_xor(rate, off, kWave);
@@ -517,32 +634,42 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
 }
 
 // One catchment: one wavefront per tile.
-template <typename T, int NACC, bool KOUT, bool QUICK>
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArgs<T> A) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int tile = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  // wave-uniform: everything derived from the tile index stays scalar
+  const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
+                                                  (int)(threadIdx.x >> 6));
   if (tile >= A.ntiles) return;
-  stage_tile<T, NACC, KOUT, QUICK>(A, tile, lane);
+  stage_tile<T, NACC, KOUT, QUICK, NANBED>(A, tile, lane);
 }
 
 // Many catchments in one launch: a persistent grid walks a tile table of
 // (member, tile) pairs; member m's arguments are margs[m].  Each tile runs
-// exactly the code of the single-catchment kernel, so a catchment's result
+// exactly the code of the single-catchment kernel for its member (the
+// coded-out path for a member with NaN bed cells), so a catchment's result
 // does not depend on which other catchments share the launch.  The table
 // length is read from device memory so that captured graphs survive changes
 // of membership.
-template <typename T, int NACC, bool QUICK>
+template <typename T, int NACC, bool QUICK, bool ANY_NANBED>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
     const StageArgs<T>* __restrict__ margs, const int2* __restrict__ tiles,
     const int* __restrict__ ntiles_p) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nwaves = gridDim.x * kWavesPerBlock;
   const int ntiles = *ntiles_p;
-  for (int k = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6); k < ntiles;
-       k += nwaves) {
+  const int k0 = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
+                                                (int)(threadIdx.x >> 6));
+  for (int k = k0; k < ntiles; k += nwaves) {
     const int m = __builtin_amdgcn_readfirstlane(tiles[k].x);
     const int t = __builtin_amdgcn_readfirstlane(tiles[k].y);
-    stage_tile<T, NACC, false, QUICK>(margs[m], t, lane);
+    if constexpr (ANY_NANBED) {
+      if (margs[m].nanbed) {
+        stage_tile<T, NACC, false, false, true>(margs[m], t, lane);
+        continue;
+      }
+    }
+    stage_tile<T, NACC, false, QUICK, false>(margs[m], t, lane);
   }
 }
 
