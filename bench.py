@@ -60,9 +60,18 @@ def device_sync():
     torch.cuda.synchronize()
 
 
-def time_precision(sc, prec, steps, warmup, stage_samples):
+def time_precision(sc, prec, steps, warmup, stage_samples, rows=0):
     from mfcm_amd import mfsv
+    from mfcm_amd.autotune import TileTuner
+    timings = None
+    if not rows:
+        tuner = TileTuner()
+        rows = tuner.tune(sc, prec)
+        timings = {k: round(v, 1) for k, v in tuner.last_timings.items()}
+        log(f"  tile rows {rows}: stage us {timings}")
     cat = mfsv.Catchment(sc, precision=prec)
+    if rows:
+        cat.set_tile_rows(rows)
     cat.run_begin()
     if warmup:
         cat.run(warmup, graph=True)
@@ -87,48 +96,69 @@ def time_precision(sc, prec, steps, warmup, stage_samples):
         "stage_ms_mean": st_ms / max(launches, 1), "stage_launches": launches,
         "sim_time_s": s1["time"], "done": s2["done"],
         "device_bytes": cat.device_bytes(),
+        "tile_rows": cat.tile_rows(), "tile_timings_us": timings,
     }
     cat.close()
     return out
 
 
 def accuracy(sc0, horizon):
-    """FP32 form vs FP64 plain after `horizon` simulated seconds."""
+    """FP32 form vs FP64 plain after `horizon` simulated seconds, beside the
+    FP64 solution's own sensitivity: the FP64 path rerun on the bed rounded
+    to FP32 precision (the input rounding the FP32 form starts from) and on
+    the bed scaled by 1 + 1e-9."""
     import copy
 
     from mfcm_amd import mfsv
     sc = copy.copy(sc0)
     sc.t_end = horizon
+    zmin = float(sc.zb.min())
+    sc_r = copy.copy(sc)
+    sc_r.zb = (sc.zb - zmin).astype(np.float32).astype(np.float64) + zmin
+    sc_p = copy.copy(sc)
+    sc_p.zb = sc.zb * (1.0 + 1e-9)
     res = {}
-    for name, prec in (("f64", mfsv.F64), ("f32", mfsv.F32)):
-        cat = mfsv.Catchment(sc, precision=prec)
+    for name, prec, s in (("f64", mfsv.F64, sc), ("f32", mfsv.F32, sc),
+                          ("f64_zb_f32", mfsv.F64, sc_r), ("f64_zb_1e-9", mfsv.F64, sc_p)):
+        cat = mfsv.Catchment(s, precision=prec)
         cat.run_begin()
         st = cat.stats()
         while not st["done"]:
             st = cat.run(4000, graph=True)
-            log(f"  accuracy {name}: t = {st['time']:.1f} s, {st['accepted']} accepted, "
-                f"{st['rejected']} rejected")
+        log(f"  accuracy {name}: t = {st['time']:.1f} s, {st['accepted']} accepted, "
+            f"{st['rejected']} rejected")
         h, u, v = cat.state()
         res[name] = (h, u, v, st)
         cat.close()
     h64, u64, v64, st64 = res["f64"]
-    h32, u32, v32, st32 = res["f32"]
     wet = h64 > 1e-3
-    vel_err = float(max(np.abs(u32 - u64)[wet].max(initial=0.0), np.abs(v32 - v64)[wet].max(initial=0.0)))
     V64 = float(np.sum(h64)) * sc.dx * sc.dy
-    V32 = float(np.sum(h32)) * sc.dx * sc.dy
+
+    def diff(name):
+        h, u, v, _ = res[name]
+        d = np.abs(h - h64)
+        V = float(np.sum(h)) * sc.dx * sc.dy
+        return {
+            "max_depth_err_m": float(d.max()),
+            "p99_depth_err_m": float(np.quantile(d, 0.99)),
+            "p999_depth_err_m": float(np.quantile(d, 0.999)),
+            "mean_depth_err_m": float(d.mean()),
+            "cells_over_1mm": int((d > 1e-3).sum()),
+            "max_velocity_err_m_s_wet": float(max(np.abs(u - u64)[wet].max(initial=0.0),
+                                                  np.abs(v - v64)[wet].max(initial=0.0))),
+            "rel_mass_err_vs_f64": abs(V - V64) / max(V64, 1e-300),
+        }
+
     mb = lambda s: (s["volume"] - s["volume0"] - s["source_volume"]) / max(abs(s["volume"]), 1e-300)
-    return {
-        "horizon_s": horizon,
-        "max_depth_err_m": float(np.abs(h32 - h64).max()),
-        "max_velocity_err_m_s_wet": vel_err,
-        "rel_mass_err_vs_f64": abs(V32 - V64) / max(V64, 1e-300),
-        "mass_balance_residual_f64": mb(st64),
-        "mass_balance_residual_f32": mb(st32),
-        "volume_m3_f64": V64,
-        "timesteps_f64": st64["accepted"] + st64["rejected"],
-        "timesteps_f32": st32["accepted"] + st32["rejected"],
-    }
+    out = {"horizon_s": horizon, "wet_cells_f64": int(wet.sum()), "volume_m3_f64": V64}
+    out.update(diff("f32"))
+    out["mass_balance_residual_f64"] = mb(st64)
+    out["mass_balance_residual_f32"] = mb(res["f32"][3])
+    out["timesteps_f64"] = st64["accepted"] + st64["rejected"]
+    out["timesteps_f32"] = res["f32"][3]["accepted"] + res["f32"][3]["rejected"]
+    out["f64_sensitivity"] = {"bed_rounded_to_f32": diff("f64_zb_f32"),
+                              "bed_scaled_1e-9": diff("f64_zb_1e-9")}
+    return out
 
 
 def ensemble(n, steps, warmup, horizon_members, horizon):
@@ -275,6 +305,102 @@ def serving(rate, duration, sides, horizons, budget_gb):
     return s
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_traffic_storm2048.json")
+
+
+def traffic_bytes(name, rows, size):
+    """Measured HBM bytes per stage launch (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE, each calibrated on a plain device copy of known size) from
+    the committed profile of this workload; None when no profile matches the
+    grid and tile height."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)[name]
+    except (OSError, KeyError, ValueError):
+        return None
+    if t.get("rows") != rows or t.get("size") != size:
+        return None
+    return t["bytes_per_launch"]
+
+
+def run_to_end(sc, prec, rows=0):
+    from mfcm_amd import mfsv
+    cat = mfsv.Catchment(sc, precision=prec)
+    if rows:
+        cat.set_tile_rows(rows)
+    cat.run_begin()
+    device_sync()
+    t0 = time.perf_counter()
+    st = cat.run(graph=True)
+    device_sync()
+    wall = time.perf_counter() - t0
+    h, u, v = cat.state()
+    cat.close()
+    return h, u, v, st, wall
+
+
+def compare(a, b, dx, dy):
+    h0, u0, v0, s0 = a[:4]
+    h1, u1, v1, s1 = b[:4]
+    d = np.abs(h1 - h0)
+    wet = h0 > 1e-3
+    V0, V1 = float(h0.sum()) * dx * dy, float(h1.sum()) * dx * dy
+    return {"max_depth_err_m": float(d.max()), "p99_depth_err_m": float(np.quantile(d, 0.99)),
+            "max_velocity_err_m_s_wet": float(max(np.abs(u1 - u0)[wet].max(initial=0.0),
+                                                  np.abs(v1 - v0)[wet].max(initial=0.0))),
+            "rel_mass_err": abs(V1 - V0) / max(V0, 1e-300),
+            "wet_masks_identical": bool(np.array_equal(h0 > 1e-6, h1 > 1e-6))}
+
+
+def hillslope_section():
+    """The CPU-runnable configuration: 256^2, dx 5 m, tilted plane + 8 hills +
+    fBm 0.5 m, dry start, 50 mm/h for 30 min, n 0.03, free outflow at x = 0,
+    Heun-2, CFL 0.5, 1 h, physics flags.  FP64 and FP32 on the GPU for the
+    whole hour (parity with the CPU oracle at this size is a GPU test)."""
+    from mfcm_amd import mfsv
+    from mfcm_amd import scenarios as S
+    sc = S.hillslope_rain(seed=0, n=256, dx=5.0, physics=True, t_end=3600.0)
+    log("hillslope 256^2, 1 h: FP64 and FP32 on the GPU")
+    r64 = run_to_end(sc, mfsv.F64)
+    r32 = run_to_end(sc, mfsv.F32)
+    out = {"grid": "256x256", "dx": 5.0, "horizon_s": 3600.0, "flags": int(sc.flags)}
+    for name, r in (("f64", r64), ("f32", r32)):
+        st = r[3]
+        out[name] = {"wall_s": r[4], "accepted": st["accepted"], "rejected": st["rejected"],
+                     "cell_updates_per_s": st["accepted"] * sc.ncells / r[4],
+                     "mass_balance_residual": (st["volume"] - st["volume0"] - st["source_volume"])
+                     / max(abs(st["volume"]), 1e-300)}
+    out["f32_vs_f64"] = compare(r64, r32, sc.dx, sc.dy)
+    return out
+
+
+def breach_section(horizon):
+    """Wet/dry front stress case: 4096^2, dx 1 m, flat bed + fBm 0.2 m, a wall
+    column at x = 1000 with a 50 m breach, 5 m upstream, dry downstream,
+    n 0.025, no rain, run to `horizon` s in FP64 and FP32."""
+    from mfcm_amd import mfsv
+    from mfcm_amd import scenarios as S
+    from mfcm_amd.autotune import TileTuner
+    sc = S.breach_front(seed=2, n=4096, dx=1.0, t_end=horizon)
+    out = {"grid": "4096x4096", "dx": 1.0, "horizon_s": horizon, "flags": int(sc.flags)}
+    res = {}
+    for name, prec in (("f32", mfsv.F32), ("f64", mfsv.F64)):
+        rows = TileTuner().tune(sc, prec, warmup=4, samples=8)
+        log(f"breach 4096^2 {name}: tile rows {rows}, running {horizon} s")
+        r = run_to_end(sc, prec, rows)
+        st = r[3]
+        res[name] = r
+        out[name] = {"wall_s": r[4], "accepted": st["accepted"], "rejected": st["rejected"],
+                     "tile_rows": rows,
+                     "cell_updates_per_s": st["accepted"] * sc.ncells / r[4],
+                     "mass_balance_residual": (st["volume"] - st["volume0"] - st["source_volume"])
+                     / max(abs(st["volume"]), 1e-300)}
+        log(f"  {name}: {out[name]['cell_updates_per_s']:.4g} cell-updates/s, "
+            f"{st['accepted']} timesteps, {r[4]:.1f} s")
+    out["f32_vs_f64"] = compare(res["f64"], res["f32"], sc.dx, sc.dy)
+    return out
+
+
 def cpu_baseline(sc, budget_s):
     """The CPU oracle on the first timesteps of the same workload."""
     from oracle import oracle as O
@@ -333,6 +459,11 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--rows-f32", type=int, default=0, help="stage tile rows (0 = autotune)")
+    ap.add_argument("--rows-f64", type=int, default=0, help="stage tile rows (0 = autotune)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 256^2 hillslope and 4096^2 breach-front sections")
+    ap.add_argument("--breach-horizon", type=float, default=600.0)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB of a plain device copy to run first (counter calibration)")
     args = ap.parse_args()
@@ -370,7 +501,8 @@ def main():
     res = {}
     for name, prec in (("f32", mfsv.F32), ("f64", mfsv.F64)):
         log(f"timing {name}: warmup {args.warmup}, steps {args.steps}")
-        res[name] = time_precision(sc, prec, args.steps, args.warmup, args.stage_samples)
+        rows = args.rows_f32 if name == "f32" else args.rows_f64
+        res[name] = time_precision(sc, prec, args.steps, args.warmup, args.stage_samples, rows)
         log(f"  {name}: {res[name]['cell_updates_per_s']:.4g} cell-updates/s, "
             f"stage {res[name]['stage_ms_mean'] * 1e3:.1f} us")
     acc = None
@@ -387,6 +519,9 @@ def main():
                       tuple(int(x) for x in args.serve_sides.split(",")),
                       tuple(float(x) for x in args.serve_horizons.split(",")),
                       args.serve_budget_gb)
+    cfgs = None
+    if not args.no_configs:
+        cfgs = {"hillslope_256": hillslope_section(), "breach_4096": breach_section(args.breach_horizon)}
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -398,7 +533,8 @@ def main():
         r = res[name]
         ach = (B[name] / 2.0) * N / (r["stage_ms_mean"] * 1e-3) / 1e9
         roof[name] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                      "frac": ach / HBM_PEAK_GBS,
+                      "traffic": traffic_bytes(name, r["tile_rows"], args.size),
                       "kernel": f"mfsv::k_stage<{'float' if name == 'f32' else 'double'}>",
                       "stage_us_mean": r["stage_ms_mean"] * 1e3,
                       "bytes_per_cell_update": B[name]}
@@ -431,6 +567,7 @@ def main():
         "accuracy": acc,
         "ensemble": ens,
         "serving": srv,
+        "configs": cfgs,
     }
     print(json.dumps(line), flush=True)
 

@@ -128,7 +128,7 @@ void geometry(mfsv_ctx* c) {
 
 int ensure_partials(mfsv_ctx* c) {
   int rc;
-  if ((rc = alloc(c, c->part_vol, sizeof(double) * c->ntiles))) return rc;
+  if ((rc = alloc(c, c->part_vol, sizeof(double) * 2 * c->ntiles))) return rc;
   if ((rc = alloc(c, c->part_rate, sizeof(double) * c->ntiles * kMaxStages))) return rc;
   return MFSV_OK;
 }

@@ -6,7 +6,7 @@ namespace mfsv {
 
 struct CtrlArgs {
   Ctrl* ctrl;
-  const double* part_vol;   // [ntiles]
+  const double* part_vol;   // [2 * ntiles]: volume, then control-number maxima
   const double* part_rate;  // [nstages][ntiles]
   int64_t ntiles;
   int32_t nstages;
@@ -20,36 +20,48 @@ struct CtrlArgs {
 constexpr int kCtrlThreads = 1024;
 constexpr int kCtrlWaves = kCtrlThreads / kWave;
 
-// Volume and per-stage source-rate sums of the tile partials, in one pass:
-// thread t accumulates tiles t, t + 1024, ... (independent accumulators),
-// then a butterfly inside each wave and a fixed-order sum over the waves.
-// Deterministic for a given tile count.  out[0] = volume partial sum,
-// out[1 + s] = stage s rate sum.
+constexpr int kCtrlVals = 2 + kMaxStages;  // volume, control number, stage rates
+
+// Volume and per-stage source-rate sums and the control-number maximum of the
+// tile partials, in one pass: thread t takes tiles t, t + 1024, ...
+// (independent accumulators), then a butterfly inside each wave and a
+// fixed-order pass over the waves.  Deterministic for a given tile count.
+// out[0] = volume partial sum, out[1] = max control number, out[2 + s] =
+// stage s rate sum.
 __device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double* out) {
-  double acc[1 + kMaxStages];
+  double acc[kCtrlVals];
 #pragma unroll
-  for (int q = 0; q <= kMaxStages; ++q) acc[q] = 0.0;
+  for (int q = 0; q < kCtrlVals; ++q) acc[q] = 0.0;
   for (int64_t k = threadIdx.x; k < A.ntiles; k += blockDim.x) {
     acc[0] += A.part_vol[k];
+    const double cn = A.part_vol[A.ntiles + k];
+    acc[1] = (cn > acc[1]) ? cn : acc[1];
 #pragma unroll
     for (int s = 0; s < kMaxStages; ++s)
-      if (s < A.nstages) acc[1 + s] += A.part_rate[(int64_t)s * A.ntiles + k];
+      if (s < A.nstages) acc[2 + s] += A.part_rate[(int64_t)s * A.ntiles + k];
   }
 #pragma unroll
-  for (int q = 0; q <= kMaxStages; ++q)
+  for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, kWave);
+    for (int q = 0; q < kCtrlVals; ++q) {
+      const double o = __shfl_xor(acc[q], off, kWave);
+      acc[q] = (q == 1) ? ((o > acc[q]) ? o : acc[q]) : acc[q] + o;
+    }
+  }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & (kWave - 1)) == 0)
 #pragma unroll
-    for (int q = 0; q <= kMaxStages; ++q) sh[w * (1 + kMaxStages) + q] = acc[q];
+    for (int q = 0; q < kCtrlVals; ++q) sh[w * kCtrlVals + q] = acc[q];
   __syncthreads();
   if (threadIdx.x == 0) {
     const int nw = (int)(blockDim.x >> 6);
 #pragma unroll
-    for (int q = 0; q <= kMaxStages; ++q) {
+    for (int q = 0; q < kCtrlVals; ++q) {
       double t = 0.0;
-      for (int i = 0; i < nw; ++i) t += sh[i * (1 + kMaxStages) + q];
+      for (int i = 0; i < nw; ++i) {
+        const double v = sh[i * kCtrlVals + q];
+        t = (q == 1) ? ((v > t) ? v : t) : t + v;
+      }
       out[q] = t;
     }
   }
@@ -60,15 +72,15 @@ __device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double*
 __device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
   Ctrl* C = A.ctrl;
   if (C->done) return;
-  double sums[1 + kMaxStages];
+  double sums[kCtrlVals];
   tile_sums(A, sh, sums);
   if (threadIdx.x != 0) return;
   const double vol = sums[0] * A.area;
   double rate = 0.0;
 #pragma unroll
   for (int s = 0; s < kMaxStages; ++s)
-    if (s < A.nstages) rate += A.b[s] * sums[1 + s];
-  const double cn_raw = __longlong_as_double((long long)C->cn_bits);
+    if (s < A.nstages) rate += A.b[s] * sums[2 + s];
+  const double cn_raw = sums[1];
   C->cn_bits = 0ull;
   C->cn = cn_raw;
   if (A.mode == 1) { C->volume = vol; return; }
@@ -115,7 +127,7 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
 }
 
 __global__ __launch_bounds__(kCtrlThreads) void k_ctrl(const CtrlArgs A) {
-  __shared__ double sh[kCtrlWaves * (1 + kMaxStages)];
+  __shared__ double sh[kCtrlWaves * kCtrlVals];
   ctrl_body(A, sh);
 }
 
@@ -123,7 +135,7 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl(const CtrlArgs A) {
 // memory so that captured graphs survive changes of membership.
 __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_batch(const CtrlArgs* __restrict__ args,
                                                              const int* __restrict__ nmembers_p) {
-  __shared__ double sh[kCtrlWaves * (1 + kMaxStages)];
+  __shared__ double sh[kCtrlWaves * kCtrlVals];
   if ((int)blockIdx.x >= *nmembers_p) return;
   ctrl_body(args[blockIdx.x], sh);
 }

@@ -49,7 +49,7 @@ struct Ctrl {
   int64_t step;         // outer steps completed
   int64_t nsteps;
   int64_t n_acc, n_rej;
-  unsigned long long cn_bits;  // atomic max of non-negative doubles
+  unsigned long long cn_bits;  // unused (the control number comes from tile partials)
   int32_t cur;          // state buffer holding U0
   int32_t done;
   int32_t repeat;
@@ -93,7 +93,7 @@ struct StageArgs {
   T* k_h; T* k_u; T* k_v;
   float* br_x; float* br_y;
   Ctrl* ctrl;
-  double* part_vol;         // [ntiles]
+  double* part_vol;         // [2 * ntiles]: volume, then control-number maxima
   double* part_rate;        // [ntiles], this stage's slice
 };
 
@@ -177,8 +177,10 @@ __device__ __forceinline__ T storm_axis(const Storm& s, double a, double c) {
     const double d = a - c;
     return exp_neg(-(d * d) / (2.0 * s.sigma * s.sigma));
   } else {
+    // accurate expf: once per column / row of a tile, and its bias would
+    // show in the FP64 rain volume
     const float d = float(a - c), sg = float(s.sigma);
-    return __expf(-(d * d) / (2.0f * sg * sg));
+    return expf(-(d * d) / (2.0f * sg * sg));
   }
 }
 
@@ -314,7 +316,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
 
   Row<T> R0 = load(y0 - 2), R1 = load(y0 - 1), R2 = load(y0);
   // Per-cell inputs of the cell pass (Manning n, accumulator bases of a
-  // later stage), loaded one row before the row that uses them.
+  // later stage).
   struct Extra {
     T n;
     T bh[NB], bu[NB], bv[NB];
@@ -334,7 +336,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     }
     return e;
   };
-  Extra X0 = load_extra(y0 - 1);
+  Extra Xn;
+  if constexpr (sizeof(T) == 8) Xn = load_extra(y0 - 1);
   // south slope quantities of the current row (rows y0-2 and y0-1 first)
   T qsh = qy(R1.h - R0.h), qsu = qy(R1.u - R0.u), qsv = qy(R1.v - R0.v), qsz = qy(R1.z - R0.z);
   // slopes of row c-1 that the cell pass and the y-face still need
@@ -346,7 +349,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
 #pragma unroll 2
   for (int c = y0 - 1; c <= y1; ++c) {
     const Row<T> Rn = load(c + 2);  // prefetch
-    const Extra X1 = load_extra(c);
+    // inputs of row c-1's cell pass: FP32 loads them here (consumed after the
+    // slopes and faces; fewer live registers), FP64 one row ahead (its
+    // occupancy is lower, so memory latency needs the longer lead)
+    Extra X0;
+    if constexpr (sizeof(T) == 4) {
+      X0 = load_extra(c - 1);
+    } else {
+      X0 = Xn;
+      Xn = load_extra(c);
+    }
     const Row<T> W = from_w(R1), E = from_e(R1);
     const bool hasN = c + 1 < ny, hasS = c > 0;
     // ---- slopes of row c (State.cpp:187-192, SpatialDerivative.hpp:104-122)
@@ -606,7 +618,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       }
     }
     R0 = R1; R1 = R2; R2 = Rn;
-    X0 = X1;
     qsh = qnh; qsu = qnu; qsv = qnv; qsz = qnz;
     p_hx = shx; p_zx = szx; p_hy = shy; p_uy = suy; p_vy = svy; p_zy = szy;
     Fs0 = Fs1;
@@ -628,7 +639,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     A.part_rate[tile] = rate;
     if (A.last) {
       A.part_vol[tile] = vol;
-      atomicMax(&C->cn_bits, (unsigned long long)__double_as_longlong(cnmax));
+      A.part_vol[A.ntiles + tile] = cnmax;  // reduced by the controller: no same-address atomics
     }
   }
 }

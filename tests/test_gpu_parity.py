@@ -156,3 +156,22 @@ def test_device_run_matches_oracle_and_mass(gpu_lib):
     resid = st["volume"] - st["volume0"] - st["source_volume"]
     assert abs(resid) <= 1e-9 * max(st["volume"], 1.0), resid
     cat.close()
+
+
+def test_hillslope_256_matches_oracle(gpu_lib):
+    """The CPU-runnable configuration at full size (256^2, dx 5 m, physics
+    flags, CFL 0.5) over its first 300 s: FP64 GPU vs the oracle."""
+    from mfcm_amd.mfsv import Catchment
+    sc = S.hillslope_rain(seed=0, n=256, dx=5.0, physics=True, t_end=300.0)
+    _say(f"hillslope 256^2 300 s flags={sc.flags}")
+    sim = _oracle(sc)
+    sim.solve()
+    cat = Catchment(sc)
+    cat.run_begin()
+    st = cat.run()
+    _compare_states(cat, sim)
+    acc, rej = sim.stats()
+    assert (st["accepted"], st["rejected"]) == (acc, rej)
+    vg, vo = cat.volume(), sim.volume(0)
+    assert abs(vg - vo) <= 1e-10 * max(abs(vo), 1e-30)
+    cat.close()
