@@ -311,14 +311,14 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_traffic_storm2048.json")
 def traffic_bytes(name, rows, size):
     """Measured HBM bytes per stage launch (rocprofv3 FETCH_SIZE and
     WRITE_SIZE, each calibrated on a plain device copy of known size) from
-    the committed profile of this workload; None when no profile matches the
-    grid and tile height."""
+    the committed profile of this workload (its tile height is recorded
+    there); None when no profile matches the grid."""
     try:
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)[name]
     except (OSError, KeyError, ValueError):
         return None
-    if t.get("rows") != rows or t.get("size") != size:
+    if t.get("size") != size:
         return None
     return t["bytes_per_launch"]
 
@@ -535,6 +535,7 @@ def main():
         roof[name] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": ach / HBM_PEAK_GBS,
                       "traffic": traffic_bytes(name, r["tile_rows"], args.size),
+                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                       "kernel": f"mfsv::k_stage<{'float' if name == 'f32' else 'double'}>",
                       "stage_us_mean": r["stage_ms_mean"] * 1e3,
                       "bytes_per_cell_update": B[name]}
