@@ -61,7 +61,17 @@ extern "C" {
 #define MFSV_FIELD_DVDT 11
 #define MFSV_FIELD_BRANCH_X 12 /* x-face branch codes, (nx+1)*ny values   */
 #define MFSV_FIELD_BRANCH_Y 13 /* y-face branch codes, nx*(ny+1) values   */
-#define MFSV_FIELD_COUNT 14
+#define MFSV_FIELD_ELOSS_FX 14 /* energy-loss coefficient along x (1/m)  */
+#define MFSV_FIELD_ELOSS_FY 15 /* energy-loss coefficient along y (1/m)  */
+#define MFSV_FIELD_INFIL_RATE 16 /* infiltration rate (m/s)              */
+#define MFSV_FIELD_INFIL_CAP 17  /* remaining infiltration capacity (m);
+                                    state: every stage consumes it       */
+#define MFSV_FIELD_SBDY0 18    /* stage boundary: water level at step start
+                                  (NaN = off)                            */
+#define MFSV_FIELD_SBDY1 19    /* stage boundary: water level at step end  */
+#define MFSV_FIELD_EDDY_MU 20  /* eddy viscosity coefficient (constant
+                                  model)                                 */
+#define MFSV_FIELD_COUNT 21
 
 typedef struct mfsv_ctx mfsv_ctx;
 
@@ -92,6 +102,20 @@ typedef struct mfsv_params {
   double storm_peak, storm_sigma, storm_cx0, storm_cy0, storm_vx, storm_vy;
   double h_dry; /* E8 dry depth (m)          */
   double u_max; /* E9 velocity cap (m/s)     */
+  /* Further source terms and the stage boundary; each 0 = off.  They run
+   * after Manning in the order energy loss, infiltration, eddy viscosity,
+   * and the stage boundary after the head boundary.
+   * energy_loss 1 (EnergyLossSourceTerm.cpp:59-88): per-cell ELOSS_FX/FY.
+   * infiltration 1 (InfiltrationSourceTerm.cpp:42-64): per-cell INFIL_RATE
+   *   and INFIL_CAP (capacity consumed at every stage).
+   * stage 1 (StageBoundarySourceTerm.cpp:43-68): per-cell SBDY0/SBDY1.
+   * eddy (EddyViscositySourceTerm.cpp:44-98, .hpp:216-256): 1 = per-cell
+   *   EDDY_MU, 2 = Smagorinsky with mu_const and mu_smag. */
+  int32_t energy_loss;
+  int32_t infiltration;
+  int32_t stage;
+  int32_t eddy;
+  double mu_const, mu_smag;
 } mfsv_params;
 
 typedef struct mfsv_stats {

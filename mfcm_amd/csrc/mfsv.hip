@@ -47,6 +47,7 @@ struct mfsv_ctx {
   Buf U[2][3];
   Buf acc[kMaxStages][3];  // accumulators for RK rows 1..nstages-1
   Buf zb, nfield, q0f, q1f, hb0f, hb1f, kf[3], brx, bry;
+  Buf fxf, fyf, iratef, icapf, sb0f, sb1f, muf;  // further sources and the stage boundary
   Buf ctrl, part_vol, part_rate, forcing, volp;
   int64_t nforcing = 0;
   Ctrl* hctrl = nullptr;  // pinned mirror
@@ -201,6 +202,14 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc)
   A.nfield = c->nfield_set ? tp<T>(c->nfield) : nullptr;
   A.q0f = tp<T>(c->q0f); A.q1f = tp<T>(c->q1f);
   A.hb0f = tp<T>(c->hb0f); A.hb1f = tp<T>(c->hb1f);
+  A.eloss = c->P.energy_loss; A.infil = c->P.infiltration;
+  A.stageb = c->P.stage; A.eddy = c->P.eddy;
+  A.mu_const = T(c->P.mu_const); A.mu_smag = T(c->P.mu_smag);
+  A.area = T(c->dx * c->dy);
+  A.fx = tp<T>(c->fxf); A.fy = tp<T>(c->fyf);
+  A.irate = tp<T>(c->iratef); A.icap = tp<T>(c->icapf);
+  A.sb0 = tp<T>(c->sb0f); A.sb1 = tp<T>(c->sb1f);
+  A.mu = tp<T>(c->muf);
   A.n_sh = T(c->P.n_shallow); A.n_dp = T(c->P.n_deep);
   A.d_sh = T(c->P.d_shallow); A.d_dp = T(c->P.d_deep);
   A.h_dry = T(c->P.h_dry); A.u_max = T(c->P.u_max);
@@ -228,16 +237,27 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc)
   return A;
 }
 
-template <typename T, bool QUICK, bool NANBED>
+template <typename T, bool QUICK, bool NANBED, bool EXT>
 void launch_q(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
   dim3 grid((unsigned)c->nblocks), block(kWave * kWavesPerBlock);
   switch (kout ? 0 : nacc) {
-    case 0: hipLaunchKernelGGL((k_stage<T, 0, true, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
-    case 1: hipLaunchKernelGGL((k_stage<T, 1, false, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
-    case 2: hipLaunchKernelGGL((k_stage<T, 2, false, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
-    case 3: hipLaunchKernelGGL((k_stage<T, 3, false, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
-    default: hipLaunchKernelGGL((k_stage<T, 4, false, QUICK, NANBED>), grid, block, 0, c->stream, A); break;
+    case 0: hipLaunchKernelGGL((k_stage<T, 0, true, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
+    case 1: hipLaunchKernelGGL((k_stage<T, 1, false, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
+    case 2: hipLaunchKernelGGL((k_stage<T, 2, false, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
+    case 3: hipLaunchKernelGGL((k_stage<T, 3, false, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
+    default: hipLaunchKernelGGL((k_stage<T, 4, false, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
   }
+}
+
+// Further source terms or the stage boundary switched on?
+bool has_ext(const mfsv_ctx* c) {
+  return c->P.energy_loss || c->P.infiltration || c->P.stage || c->P.eddy;
+}
+
+template <typename T, bool QUICK, bool NANBED>
+void launch_x(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
+  if (has_ext(c)) launch_q<T, QUICK, NANBED, true>(c, A, nacc, kout);
+  else launch_q<T, QUICK, NANBED, false>(c, A, nacc, kout);
 }
 
 // True when x is a power of two: x/d == x * (1/d) exactly.
@@ -254,12 +274,12 @@ void launch_typed(mfsv_ctx* c, int s, bool kout, bool branches) {
   int nacc = 0;
   const StageArgs<T> A = stage_args<T>(c, s, kout, branches, &nacc);
   if (c->nanbed) {
-    launch_q<T, false, true>(c, A, nacc, kout);
+    launch_x<T, false, true>(c, A, nacc, kout);
   } else if constexpr (sizeof(T) == 4) {
-    launch_q<T, true, false>(c, A, nacc, kout);
+    launch_x<T, true, false>(c, A, nacc, kout);
   } else {
-    if (pow2(c->dx) && pow2(c->dy)) launch_q<T, true, false>(c, A, nacc, kout);
-    else launch_q<T, false, false>(c, A, nacc, kout);
+    if (pow2(c->dx) && pow2(c->dy)) launch_x<T, true, false>(c, A, nacc, kout);
+    else launch_x<T, false, false>(c, A, nacc, kout);
   }
 }
 
@@ -446,6 +466,8 @@ int mfsv_destroy(mfsv_ctx* c) {
   drop_graph(c);
   for (auto& a : c->U) for (auto& b : a) release(b);
   for (auto& a : c->acc) for (auto& b : a) release(b);
+  for (Buf* b : {&c->fxf, &c->fyf, &c->iratef, &c->icapf, &c->sb0f, &c->sb1f, &c->muf})
+    release(*b);
   for (Buf* b : {&c->zb, &c->nfield, &c->q0f, &c->q1f, &c->hb0f, &c->hb1f, &c->kf[0], &c->kf[1],
                  &c->kf[2], &c->brx, &c->bry, &c->ctrl, &c->part_vol, &c->part_rate, &c->forcing,
                  &c->volp})
@@ -478,6 +500,9 @@ int mfsv_set_params(mfsv_ctx* c, const mfsv_params* p) {
   if (!c || !p) return MFSV_ERR_ARG;
   if (p->discharge < 0 || p->discharge > 2 || p->head < 0 || p->head > 2)
     return fail(c, MFSV_ERR_ARG, "discharge/head mode out of range");
+  if (p->energy_loss < 0 || p->energy_loss > 1 || p->infiltration < 0 || p->infiltration > 1 ||
+      p->stage < 0 || p->stage > 1 || p->eddy < 0 || p->eddy > 2)
+    return fail(c, MFSV_ERR_ARG, "source-term mode out of range");
   if (!(p->cn_target > 0)) return fail(c, MFSV_ERR_ARG, "cn_target must be positive");
   if ((p->flags & MFSV_E4_STORM) && !(p->storm_sigma > 0))
     return fail(c, MFSV_ERR_ARG, "storm sigma must be positive");
@@ -597,6 +622,14 @@ int mfsv_upload(mfsv_ctx* c, int field, const double* host, int64_t n) {
     case MFSV_FIELD_QBDY1: return put(c, c->q1f, host, n, 0.0);
     case MFSV_FIELD_HBDY0: return put(c, c->hb0f, host, n, 0.0);
     case MFSV_FIELD_HBDY1: return put(c, c->hb1f, host, n, 0.0);
+    case MFSV_FIELD_ELOSS_FX: return put(c, c->fxf, host, n, 0.0);
+    case MFSV_FIELD_ELOSS_FY: return put(c, c->fyf, host, n, 0.0);
+    case MFSV_FIELD_INFIL_RATE: return put(c, c->iratef, host, n, 0.0);
+    case MFSV_FIELD_INFIL_CAP: return put(c, c->icapf, host, n, 0.0);
+    // water levels, relative to the bed offset of the FP32 form
+    case MFSV_FIELD_SBDY0: return put(c, c->sb0f, host, n, c->zoff);
+    case MFSV_FIELD_SBDY1: return put(c, c->sb1f, host, n, c->zoff);
+    case MFSV_FIELD_EDDY_MU: return put(c, c->muf, host, n, 0.0);
     default: return fail(c, MFSV_ERR_ARG, "field is not uploadable");
   }
 }
@@ -626,6 +659,13 @@ int mfsv_download(mfsv_ctx* c, int field, double* host, int64_t n) {
     case MFSV_FIELD_QBDY1: return get(c, c->q1f, host, n, 0.0, c->vsz);
     case MFSV_FIELD_HBDY0: return get(c, c->hb0f, host, n, 0.0, c->vsz);
     case MFSV_FIELD_HBDY1: return get(c, c->hb1f, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_ELOSS_FX: return get(c, c->fxf, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_ELOSS_FY: return get(c, c->fyf, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_INFIL_RATE: return get(c, c->iratef, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_INFIL_CAP: return get(c, c->icapf, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_SBDY0: return get(c, c->sb0f, host, n, c->zoff, c->vsz);
+    case MFSV_FIELD_SBDY1: return get(c, c->sb1f, host, n, c->zoff, c->vsz);
+    case MFSV_FIELD_EDDY_MU: return get(c, c->muf, host, n, 0.0, c->vsz);
     case MFSV_FIELD_DHDT: case MFSV_FIELD_DUDT: case MFSV_FIELD_DVDT:
       return get(c, c->kf[field - MFSV_FIELD_DHDT], host, n, 0.0, c->vsz);
     default: return MFSV_ERR_ARG;
@@ -635,6 +675,14 @@ int mfsv_download(mfsv_ctx* c, int field, double* host, int64_t n) {
 static int check_inputs(mfsv_ctx* c) {
   if (c->P.discharge == 2 && (!c->q0f.p || !c->q1f.p))
     return fail(c, MFSV_ERR_STATE, "per-cell discharge needs QBDY0 and QBDY1");
+  if (c->P.energy_loss && (!c->fxf.p || !c->fyf.p))
+    return fail(c, MFSV_ERR_STATE, "energy loss needs ELOSS_FX and ELOSS_FY");
+  if (c->P.infiltration && (!c->iratef.p || !c->icapf.p))
+    return fail(c, MFSV_ERR_STATE, "infiltration needs INFIL_RATE and INFIL_CAP");
+  if (c->P.stage && (!c->sb0f.p || !c->sb1f.p))
+    return fail(c, MFSV_ERR_STATE, "the stage boundary needs SBDY0 and SBDY1");
+  if (c->P.eddy == 1 && !c->muf.p)
+    return fail(c, MFSV_ERR_STATE, "constant eddy viscosity needs EDDY_MU");
   if (c->P.head == 2 && (!c->hb0f.p || !c->hb1f.p))
     return fail(c, MFSV_ERR_STATE, "per-cell head needs HBDY0 and HBDY1");
   return MFSV_OK;

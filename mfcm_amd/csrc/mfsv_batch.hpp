@@ -27,7 +27,7 @@ struct mfsv_batch {
   int ntiles = 0;
   bool dirty = true;
   bool quick = true;
-  bool any_nanbed = false;         // a member has coded-out cells
+  bool any_nanbed = false;         // a member has coded-out cells or further sources
   int nblocks = 0;                 // persistent grid
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -114,8 +114,8 @@ int upload_tables(mfsv_batch* b) {
   bool quick = true;  // exact division stays exact for power-of-two spacing
   if (sizeof(T) == 8)
     for (auto* c : b->active) quick = quick && pow2(c->dx) && pow2(c->dy);
-  bool any_nanbed = false;
-  for (auto* c : b->active) any_nanbed = any_nanbed || c->nanbed;
+  bool any_nanbed = false;  // a member needs the coded-out or further-sources path
+  for (auto* c : b->active) any_nanbed = any_nanbed || c->nanbed || has_ext(c);
   if (quick != b->quick || any_nanbed != b->any_nanbed) bdrop_graph(b);
   b->quick = quick;
   b->any_nanbed = any_nanbed;
@@ -149,17 +149,17 @@ int upload_tables(mfsv_batch* b) {
   return MFSV_OK;
 }
 
-template <typename T, bool QUICK, bool ANY_NANBED>
+template <typename T, bool QUICK, bool GENERAL>
 void launch_batch_stage(mfsv_batch* b, int s) {
   const int* cnt = static_cast<const int*>(b->counts.p);
   const StageArgs<T>* m = static_cast<const StageArgs<T>*>(b->margs[s].p);
   const int2* t = static_cast<const int2*>(b->tiles.p);
   dim3 grid((unsigned)b->nblocks), block(kWave * kWavesPerBlock);
   switch (b->nstages - s) {
-    case 1: hipLaunchKernelGGL((k_stage_batch<T, 1, QUICK, ANY_NANBED>), grid, block, 0, b->stream, m, t, cnt); break;
-    case 2: hipLaunchKernelGGL((k_stage_batch<T, 2, QUICK, ANY_NANBED>), grid, block, 0, b->stream, m, t, cnt); break;
-    case 3: hipLaunchKernelGGL((k_stage_batch<T, 3, QUICK, ANY_NANBED>), grid, block, 0, b->stream, m, t, cnt); break;
-    default: hipLaunchKernelGGL((k_stage_batch<T, 4, QUICK, ANY_NANBED>), grid, block, 0, b->stream, m, t, cnt); break;
+    case 1: hipLaunchKernelGGL((k_stage_batch<T, 1, QUICK, GENERAL>), grid, block, 0, b->stream, m, t, cnt); break;
+    case 2: hipLaunchKernelGGL((k_stage_batch<T, 2, QUICK, GENERAL>), grid, block, 0, b->stream, m, t, cnt); break;
+    case 3: hipLaunchKernelGGL((k_stage_batch<T, 3, QUICK, GENERAL>), grid, block, 0, b->stream, m, t, cnt); break;
+    default: hipLaunchKernelGGL((k_stage_batch<T, 4, QUICK, GENERAL>), grid, block, 0, b->stream, m, t, cnt); break;
   }
 }
 

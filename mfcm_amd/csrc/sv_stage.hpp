@@ -95,6 +95,13 @@ struct StageArgs {
   Ctrl* ctrl;
   double* part_vol;         // [2 * ntiles]: volume, then control-number maxima
   double* part_rate;        // [ntiles], this stage's slice
+  // further source terms (0 = off) and their per-cell fields
+  int32_t eloss, infil, stageb, eddy;
+  T mu_const, mu_smag, area;
+  const T* fx; const T* fy;
+  const T* irate; T* icap;
+  const T* sb0; const T* sb1;
+  const T* mu;
 };
 
 template <typename T> struct Row { T h, u, v, z; };
@@ -211,7 +218,9 @@ __device__ __forceinline__ Side<T> outflow_ghost(const Side<T>& c) {
 // face projection (d/2) * slope is exactly m/2, and -g * slope is exactly
 // m * (-g/d), so every value is the reference's bit for bit.  Otherwise
 // (and with coded-out cells, NANBED) slopes are exact quotients.
-template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED>
+// EXT: the further source terms (energy loss, infiltration, eddy viscosity)
+// and the stage boundary are compiled in (each still switched at run time).
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT>
 __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile, const int lane) {
   static_assert(!(QUICK && NANBED), "coded-out cells take the quotient path");
   constexpr bool kRel = sizeof(T) == 4;
@@ -342,6 +351,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   T qsh = qy(R1.h - R0.h), qsu = qy(R1.u - R0.u), qsv = qy(R1.v - R0.v), qsz = qy(R1.z - R0.z);
   // slopes of row c-1 that the cell pass and the y-face still need
   T p_hx = T(0), p_zx = T(0), p_hy = T(0), p_uy = T(0), p_vy = T(0), p_zy = T(0);
+  // EXT: x slopes of u, v of row c-1 and y slopes of u, v of row c-2 (the
+  // second derivatives of the eddy-viscosity term)
+  T p_ux = T(0), p_vx = T(0), pp_uy = T(0), pp_vy = T(0);
   FaceUV<T> Fs0 = {}, Xw0 = {};
   double vol = 0.0, rate = 0.0;
   T qmax = T(0);
@@ -523,6 +535,77 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         dudt = wet ? dudt + dum : dudt;
         dvdt = wet ? dvdt + dvm : dvdt;
       }
+      if constexpr (EXT) {
+        // overshoot cap shared by the velocity sources: a change that would
+        // reverse the velocity within the timestep becomes -vel/dt
+        auto vcap = [&](T d, T vel) -> T {
+          const T p = d * dtT;
+          return (fabs_t(p) > fabs_t(vel) && sy_sign(p) != sy_sign(vel)) ? div_dt(-vel, dtT, rdt) : d;
+        };
+        const bool wet = h > T(1e-6);
+        // EnergyLossSourceTerm.cpp:59-88
+        if (A.eloss) {
+          const T hyp = fsqrt_t(u * u + v * v);
+          const T due = vcap(T(-0.5) * u * ld(A.fx, wo) * hyp, u);
+          const T dve = vcap(T(-0.5) * v * ld(A.fy, wo) * hyp, v);
+          dudt = wet ? dudt + due : dudt;
+          dvdt = wet ? dvdt + dve : dvdt;
+        }
+        // InfiltrationSourceTerm.cpp:42-64 (the capacity is consumed here)
+        if (A.infil) {
+          T di = ld(A.irate, wo) * dtT;
+          di = (di > h) ? h : di;
+          const T capv = ld(A.icap, wo);
+          di = (di > capv) ? capv : di;
+          dhdt -= div_dt(di, dtT, rdt);
+          if (own) st(A.icap, wo, T(capv - di));
+        }
+        // EddyViscositySourceTerm.cpp:44-98: Minmod3 of the first-derivative
+        // fields (x of du/dx, y of du/dy), Smagorinsky or per-cell mu
+        if (A.eddy) {
+          const T Wu = from_w(p_ux), Eu = from_e(p_ux), Wv = from_w(p_vx), Ev = from_e(p_vx);
+          const bool hS = r > 0, hN = r + 1 < ny;
+          T d2ux, d2vx, d2uy, d2vy;
+          if constexpr (!NANBED) {
+            d2ux = minmod_q(qx(p_ux - Wu), qx(Eu - p_ux), qx(Eu - Wu), hasW, hasE);
+            d2vx = minmod_q(qx(p_vx - Wv), qx(Ev - p_vx), qx(Ev - Wv), hasW, hasE);
+            const T yu = minmod_q(qy(p_uy - pp_uy), qy(suy - p_uy), qy(suy - pp_uy), hS, hN);
+            const T yv = minmod_q(qy(p_vy - pp_vy), qy(svy - p_vy), qy(svy - pp_vy), hS, hN);
+            d2uy = e7 ? yu : -yu;
+            d2vy = e7 ? yv : -yv;
+            if constexpr (QUICK) {
+              d2ux = d2ux * rdx * rdx; d2vx = d2vx * rdx * rdx;
+              d2uy = d2uy * rdy * rdy; d2vy = d2vy * rdy * rdy;
+            }
+          } else {
+            d2ux = minmod3(hasW ? Wu : p_ux, hasW, p_ux, hasE, hasE ? Eu : p_ux, dx, rdx);
+            d2vx = minmod3(hasW ? Wv : p_vx, hasW, p_vx, hasE, hasE ? Ev : p_vx, dx, rdx);
+            const T Nu = hN ? suy : p_uy, Su = hS ? pp_uy : p_uy;
+            const T Nv = hN ? svy : p_vy, Sv = hS ? pp_vy : p_vy;
+            if (e7) {
+              d2uy = minmod3(Su, hS, p_uy, hN, Nu, dy, rdy);
+              d2vy = minmod3(Sv, hS, p_vy, hN, Nv, dy, rdy);
+            } else {
+              d2uy = minmod3(Nu, hN, p_uy, hS, Su, dy, rdy);
+              d2vy = minmod3(Nv, hN, p_vy, hS, Sv, dy, rdy);
+            }
+          }
+          T mu;
+          if (A.eddy == 2) {
+            T dudx_ = p_ux, dvdy_ = p_vy, dudy_ = p_uy, dvdx_ = p_vx;
+            if constexpr (QUICK) { dudx_ *= rdx; dvdx_ *= rdx; dudy_ *= rdy; dvdy_ *= rdy; }
+            const T t = fabs_t(dudy_) + fabs_t(dvdx_);
+            mu = A.mu_const + A.mu_smag * A.area *
+                 fsqrt_t(dudx_ * dudx_ + dvdy_ * dvdy_ + T(0.5) * (t * t));
+          } else {
+            mu = ld(A.mu, wo);
+          }
+          const T due = vcap(-mu * (d2ux + d2uy), u);
+          const T dve = vcap(-mu * (d2vx + d2vy), v);
+          dudt = wet ? dudt + due : dudt;
+          dvdt = wet ? dvdt + dve : dvdt;
+        }
+      }
       // DischargeBoundarySourceTerm.cpp:43-67 (E1 flips the guard)
       if (A.discharge || storm) {
         T tq0, tq1;
@@ -573,6 +656,26 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         target = (target <= T(0)) ? T(0) : target;
         dhdt = (!(h0 != h0) && !(h1 != h1)) ? div_dt(target - h, dtT, rdt) : dhdt;
       }
+      // StageBoundarySourceTerm.cpp:43-68: the head boundary on level - bed
+      if constexpr (EXT) {
+        if (A.stageb) {
+          const T h0 = ld(A.sb0, wo) - R0.z, h1 = ld(A.sb1, wo) - R0.z;
+          T target;
+          if constexpr (sizeof(T) == 8) {
+            const double dhb = div_c(h1 - h0, step_len, A.rstep);
+            const double now = h0 + tsub * dhb;
+            const double next = now + dt * dhb;
+            target = 0.5 * (now + next);
+          } else {
+            const float dhb = (h1 - h0) * float(A.rstep);
+            const float now = __builtin_fmaf(float(tsub), dhb, h0);
+            const float next = __builtin_fmaf(dtT, dhb, now);
+            target = 0.5f * (now + next);
+          }
+          target = (target <= T(0)) ? T(0) : target;
+          dhdt = (!(h0 != h0) && !(h1 != h1)) ? div_dt(target - h, dtT, rdt) : dhdt;
+        }
+      }
       rate += own ? double(dhdt) : 0.0;
       if constexpr (KOUT) {
         if (own) {
@@ -619,6 +722,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     }
     R0 = R1; R1 = R2; R2 = Rn;
     qsh = qnh; qsu = qnu; qsv = qnv; qsz = qnz;
+    if constexpr (EXT) { pp_uy = p_uy; pp_vy = p_vy; p_ux = sux; p_vx = svx; }
     p_hx = shx; p_zx = szx; p_hy = shy; p_uy = suy; p_vy = svy; p_zy = szy;
     Fs0 = Fs1;
     Xw0 = Xw1;
@@ -645,24 +749,25 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
 }
 
 // One catchment: one wavefront per tile.
-template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED>
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArgs<T> A) {
   const int lane = threadIdx.x & (kWave - 1);
   // wave-uniform: everything derived from the tile index stays scalar
   const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
                                                   (int)(threadIdx.x >> 6));
   if (tile >= A.ntiles) return;
-  stage_tile<T, NACC, KOUT, QUICK, NANBED>(A, tile, lane);
+  stage_tile<T, NACC, KOUT, QUICK, NANBED, EXT>(A, tile, lane);
 }
 
 // Many catchments in one launch: a persistent grid walks a tile table of
 // (member, tile) pairs; member m's arguments are margs[m].  Each tile runs
-// exactly the code of the single-catchment kernel for its member (the
-// coded-out path for a member with NaN bed cells), so a catchment's result
+// exactly the code of the single-catchment kernel for its member (with
+// GENERAL, the coded-out or further-sources variant where the member needs
+// it), so a catchment's result
 // does not depend on which other catchments share the launch.  The table
 // length is read from device memory so that captured graphs survive changes
 // of membership.
-template <typename T, int NACC, bool QUICK, bool ANY_NANBED>
+template <typename T, int NACC, bool QUICK, bool GENERAL>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
     const StageArgs<T>* __restrict__ margs, const int2* __restrict__ tiles,
     const int* __restrict__ ntiles_p) {
@@ -674,13 +779,19 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
   for (int k = k0; k < ntiles; k += nwaves) {
     const int m = __builtin_amdgcn_readfirstlane(tiles[k].x);
     const int t = __builtin_amdgcn_readfirstlane(tiles[k].y);
-    if constexpr (ANY_NANBED) {
-      if (margs[m].nanbed) {
-        stage_tile<T, NACC, false, false, true>(margs[m], t, lane);
+    if constexpr (GENERAL) {
+      const StageArgs<T>& a = margs[m];
+      const bool ext = (a.eloss | a.infil | a.stageb | a.eddy) != 0;
+      if (a.nanbed) {
+        stage_tile<T, NACC, false, false, true, true>(a, t, lane);
+        continue;
+      }
+      if (ext) {
+        stage_tile<T, NACC, false, QUICK, false, true>(a, t, lane);
         continue;
       }
     }
-    stage_tile<T, NACC, false, QUICK, false>(margs[m], t, lane);
+    stage_tile<T, NACC, false, QUICK, false, false>(margs[m], t, lane);
   }
 }
 

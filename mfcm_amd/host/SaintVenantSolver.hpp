@@ -71,6 +71,18 @@ struct SaintVenantOptions {
   mfsv_params storm{};            // storm_* fields used when flags has MFSV_E4_STORM
   double h_dry = 1e-3, u_max = 10.0;
   int tile_rows = 0;              // 0 = library default
+  // EnergyLossSourceTerm (per-cell fx, fy; empty = off)
+  std::vector<double> eloss_fx, eloss_fy;
+  // InfiltrationSourceTerm (per-cell rate and initial capacity; empty = off)
+  std::vector<double> infil_rate, infil_capacity;
+  // EddyViscositySourceTerm: 0 off, 1 per-cell eddy_mu, 2 Smagorinsky
+  int eddy = 0;
+  std::vector<double> eddy_mu;
+  double mu_const = 0.05, mu_smag = 0.5;
+  // StageBoundarySourceTerm: water-level series on the cells where
+  // stage_cells is non-zero (empty series = off)
+  TimeSeries stage;
+  std::vector<uint8_t> stage_cells;
 };
 
 template <typename TT, typename T>
@@ -113,6 +125,11 @@ class SaintVenantSolver {
     p.head = opt.head_cols > 0 ? 1 : 0;
     p.head_x0 = 0; p.head_x1 = opt.head_cols;
     p.h_dry = opt.h_dry; p.u_max = opt.u_max;
+    p.energy_loss = opt.eloss_fx.empty() ? 0 : 1;
+    p.infiltration = opt.infil_rate.empty() ? 0 : 1;
+    p.stage = opt.stage.empty() ? 0 : 1;
+    p.eddy = opt.eddy;
+    p.mu_const = opt.mu_const; p.mu_smag = opt.mu_smag;
     check(mfsv_set_params(c, &p), c, "mfsv_set_params");
     check(mfsv_set_time(c, tparams->start_time(), tparams->end_time(), tparams->step_duration(),
                         tparams->initial_timestep()), c, "mfsv_set_time");
@@ -123,6 +140,17 @@ class SaintVenantSolver {
       DataArray<double> nf(n);
       bind_and_upload(nf, MFSV_FIELD_MANNING_N, opt.manning_n);
     }
+    auto extra = [&](const std::vector<double>& f, int id) {
+      if (f.empty()) return;
+      if (f.size() != n) throw std::runtime_error("source-term field size != cell count");
+      DataArray<double> a(n);
+      bind_and_upload(a, id, f);
+    };
+    extra(opt.eloss_fx, MFSV_FIELD_ELOSS_FX);
+    extra(opt.eloss_fy, MFSV_FIELD_ELOSS_FY);
+    extra(opt.infil_rate, MFSV_FIELD_INFIL_RATE);
+    extra(opt.infil_capacity, MFSV_FIELD_INFIL_CAP);
+    extra(opt.eddy_mu, MFSV_FIELD_EDDY_MU);
     bind_and_upload(U_.h.data(), MFSV_FIELD_H, h0);
     bind_and_upload(U_.u.data(), MFSV_FIELD_U, u0);
     bind_and_upload(U_.v.data(), MFSV_FIELD_V, v0);
@@ -166,6 +194,20 @@ class SaintVenantSolver {
     if (!opt_.rain.empty()) {
       q0 = 0.0 + opt_.rain.at(double(time_now));
       q1 = 0.0 + opt_.rain.at(double(time_now + tparams_->step_duration()));
+    }
+    if (!opt_.stage.empty()) {  // StageBoundarySourceTerm.hpp:114-142
+      const double nan = std::numeric_limits<double>::quiet_NaN();
+      const double a0 = opt_.stage.at(double(time_now));
+      const double a1 = opt_.stage.at(double(time_now + tparams_->step_duration()));
+      const size_t nc = mesh_.cell_count();
+      std::vector<double> s0(nc), s1(nc);
+      for (size_t i = 0; i < nc; ++i) {
+        const bool on = opt_.stage_cells.empty() || opt_.stage_cells[i];
+        s0[i] = on ? a0 : nan;
+        s1[i] = on ? a1 : nan;
+      }
+      check(mfsv_upload(ctx_->get(), MFSV_FIELD_SBDY0, s0.data(), int64_t(nc)), ctx_->get(), "mfsv_upload");
+      check(mfsv_upload(ctx_->get(), MFSV_FIELD_SBDY1, s1.data(), int64_t(nc)), ctx_->get(), "mfsv_upload");
     }
     const double hb = opt_.head_cols > 0 ? 0.0 : std::numeric_limits<double>::quiet_NaN();
     check(mfsv_start_new_step(ctx_->get(), double(time_now), q0, q1, hb, hb), ctx_->get(),

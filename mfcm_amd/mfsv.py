@@ -29,7 +29,8 @@ E1_RAIN_GUARD, E2_WEST_OUTFLOW, E4_STORM, E7_Y_ORIENT = 0x01, 0x02, 0x08, 0x40
 E8_DRY_VELOCITY, E9_VELOCITY_CAP = 0x80, 0x100
 FIELD = {"h": 0, "u": 1, "v": 2, "zb": 3, "manning_n": 4, "qbdy0": 5, "qbdy1": 6,
          "hbdy0": 7, "hbdy1": 8, "dhdt": 9, "dudt": 10, "dvdt": 11,
-         "branch_x": 12, "branch_y": 13}
+         "branch_x": 12, "branch_y": 13, "eloss_fx": 14, "eloss_fy": 15,
+         "infil_rate": 16, "infil_cap": 17, "sbdy0": 18, "sbdy1": 19, "eddy_mu": 20}
 
 ERRORS = {-1: "bad argument", -2: "HIP runtime error", -3: "invalid state",
           -4: "out of device memory", -5: "unsupported"}
@@ -72,7 +73,10 @@ class Params(C.Structure):
                 ("storm_peak", C.c_double), ("storm_sigma", C.c_double),
                 ("storm_cx0", C.c_double), ("storm_cy0", C.c_double),
                 ("storm_vx", C.c_double), ("storm_vy", C.c_double),
-                ("h_dry", C.c_double), ("u_max", C.c_double)]
+                ("h_dry", C.c_double), ("u_max", C.c_double),
+                ("energy_loss", C.c_int32), ("infiltration", C.c_int32),
+                ("stage", C.c_int32), ("eddy", C.c_int32),
+                ("mu_const", C.c_double), ("mu_smag", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -220,12 +224,32 @@ class Catchment:
             p.storm_cx0, p.storm_cy0, p.storm_vx, p.storm_vy = st.cx0, st.cy0, st.vx, st.vy
         p.h_dry = float(getattr(sc, "h_dry", 1e-3))
         p.u_max = float(getattr(sc, "u_max", 10.0))
+        cells = lambda a: np.broadcast_to(np.asarray(a, np.float64), (self.ny, self.nx))
+        if getattr(sc, "energy_loss", None) is not None:
+            p.energy_loss = 1
+            self.upload("eloss_fx", cells(sc.energy_loss[0]))
+            self.upload("eloss_fy", cells(sc.energy_loss[1]))
+        if getattr(sc, "infiltration", None) is not None:
+            p.infiltration = 1
+            self.upload("infil_rate", cells(sc.infiltration[0]))
+            self.upload("infil_cap", cells(sc.infiltration[1]))
+        if getattr(sc, "eddy", None) is not None:
+            if "mu" in sc.eddy:
+                p.eddy = 1
+                self.upload("eddy_mu", cells(sc.eddy["mu"]))
+            else:
+                p.eddy = 2
+                p.mu_const, p.mu_smag = (float(x) for x in sc.eddy["smagorinsky"])
+        if getattr(sc, "stage_t", None) is not None:
+            p.stage = 1
         self.params = p
         self._check(lib().mfsv_set_params(self._h, C.byref(p)))
         self._check(lib().mfsv_set_time(self._h, 0.0, float(sc.t_end), float(sc.step_duration),
                                          float(sc.initial_dt)))
         self.upload("zb", sc.zb)
         self.set_state(sc.h0, sc.u0, sc.v0)
+        if p.stage:
+            self.set_stage_levels(0.0)
         self.nsteps = num_steps(0.0, float(sc.t_end), float(sc.step_duration))
         self._table = self.forcing_table()
         self._check(lib().mfsv_set_forcing_table(self._h, self.nsteps, _ptr(self._table)))
@@ -294,6 +318,17 @@ class Catchment:
         self._check(lib().mfsv_device_bytes(self._h, C.byref(b)))
         return b.value
 
+    def set_stage_levels(self, time_now):
+        """StageBoundarySourceTerm.hpp:114-142: water-level endpoints of the
+        outer step starting at time_now on the stage cells (NaN elsewhere)."""
+        sc = self.sc
+        sel = np.asarray(sc.stage_sel, bool) if sc.stage_sel is not None else np.ones(
+            (self.ny, self.nx), bool)
+        a0 = ts_linterp(sc.stage_t, sc.stage_v, time_now)
+        a1 = ts_linterp(sc.stage_t, sc.stage_v, time_now + sc.step_duration)
+        self.upload("sbdy0", np.where(sel, a0, np.nan))
+        self.upload("sbdy1", np.where(sel, a1, np.nan))
+
     # -- forcing (Solver.cpp:119-129 start_new_step) ----------------------
     def forcing_at(self, time_now):
         sc = self.sc
@@ -314,6 +349,8 @@ class Catchment:
 
     # -- reference-shaped host-driven loop --------------------------------
     def start_new_step(self, time_now):
+        if self.params.stage:
+            self.set_stage_levels(time_now)
         q0, q1, hb0, hb1 = self.forcing_at(time_now)
         self._check(lib().mfsv_start_new_step(self._h, time_now, q0, q1, hb0, hb1))
 

@@ -131,6 +131,13 @@ class Scenario:
     cn_target: float = 1.0
     h_dry: float = 1e-3                  # E8: no velocity where h <= h_dry
     u_max: float = 10.0                  # E9: velocity components within +-u_max
+    # further source terms and boundaries (None = off):
+    energy_loss: tuple | None = None      # (fx, fy) per metre, scalars or per-cell arrays
+    infiltration: tuple | None = None     # (rate m/s, initial capacity m)
+    eddy: dict | None = None              # {"mu": value/array} or {"smagorinsky": (c, s)}
+    stage_t: np.ndarray | None = None     # stage boundary: water-level series (s, m)
+    stage_v: np.ndarray | None = None
+    stage_sel: np.ndarray | None = None   # cells of the stage boundary (ny x nx, bool)
     t_end: float = 3600.0
     step_duration: float = 60.0
     initial_dt: float = 1.0

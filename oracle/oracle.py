@@ -100,6 +100,10 @@ def lib():
         L.or_sim_add_discharge.argtypes = [p, C.c_int, p, p, p]
         L.or_sim_add_head.argtypes = [p, C.c_int, p, p, p]
         L.or_sim_set_storm.argtypes = [p, C.POINTER(Storm), d, d]
+        L.or_sim_add_stage.argtypes = [p, C.c_int, p, p, p]
+        L.or_sim_set_energy_loss.argtypes = [p, p, p]
+        L.or_sim_set_infiltration.argtypes = [p, p, p]
+        L.or_sim_set_eddy_viscosity.argtypes = [p, p, d, d]
         L.or_sim_set_time.argtypes = [p, d, d, d, d]
         L.or_sim_set_dry_depth.argtypes = [p, d]
         L.or_sim_set_velocity_cap.argtypes = [p, d]
@@ -177,7 +181,9 @@ ARR = {"h": 0, "u": 1, "v": 2, "dhdt": 3, "dudt": 4, "dvdt": 5,
        "dhdx": 6, "dudx": 7, "dvdx": 8, "dhdy": 9, "dudy": 10, "dvdy": 11,
        "zb": 12, "dzbdx": 13, "dzbdy": 14, "hflux": 15, "uflux": 16,
        "vflux": 17, "zflux": 18, "branch": 19, "qbdy0": 20, "qbdy1": 21,
-       "hbdy0": 22, "hbdy1": 23, "mannings_n": 24, "friction_slope": 25}
+       "hbdy0": 22, "hbdy1": 23, "mannings_n": 24, "friction_slope": 25,
+       "infil_capacity": 26, "eddy_mu": 27, "d2udx2": 28, "d2udy2": 29, "d2vdx2": 30,
+       "d2vdy2": 31, "sbdy0": 32, "sbdy1": 33}
 
 
 class Sim:
@@ -241,6 +247,26 @@ class Sim:
 
     def add_head(self, t, v, sel=None):
         self._series(lib().or_sim_add_head, t, v, sel)
+
+    def add_stage(self, t, v, sel=None):
+        self._series(lib().or_sim_add_stage, t, v, sel)
+
+    def _cells(self, a):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64),
+                                                    (self.ny, self.nx))).ravel()
+
+    def set_energy_loss(self, fx, fy):
+        fx, fy = self._cells(fx), self._cells(fy)
+        lib().or_sim_set_energy_loss(self._h, _ptr(fx), _ptr(fy))
+
+    def set_infiltration(self, rate, capacity):
+        r, c = self._cells(rate), self._cells(capacity)
+        lib().or_sim_set_infiltration(self._h, _ptr(r), _ptr(c))
+
+    def set_eddy_viscosity(self, mu=None, mu_const=0.05, mu_smag=0.5):
+        m = None if mu is None else self._cells(mu)
+        lib().or_sim_set_eddy_viscosity(self._h, None if m is None else _ptr(m),
+                                        float(mu_const), float(mu_smag))
 
     def set_storm(self, storm: dict, origin=(0.0, 0.0)):
         s = Storm(**storm)
@@ -335,6 +361,18 @@ def sim_from_scenario(sc):
         sim.add_head([0.0, 1e12], [0.0, 0.0], sel)
     if sc.storm is not None:
         sim.set_storm(sc.storm.as_dict())
+    if getattr(sc, "energy_loss", None) is not None:
+        sim.set_energy_loss(*sc.energy_loss)
+    if getattr(sc, "infiltration", None) is not None:
+        sim.set_infiltration(*sc.infiltration)
+    if getattr(sc, "eddy", None) is not None:
+        e = sc.eddy
+        if "mu" in e:
+            sim.set_eddy_viscosity(e["mu"])
+        else:
+            sim.set_eddy_viscosity(None, *e["smagorinsky"])
+    if getattr(sc, "stage_t", None) is not None:
+        sim.add_stage(sc.stage_t, sc.stage_v, sc.stage_sel)
     sim.set_time(0.0, sc.t_end, sc.step_duration, sc.initial_dt)
     sim.set_dry_depth(sc.h_dry)
     sim.set_velocity_cap(sc.u_max)

@@ -316,6 +316,16 @@ struct or_sim {
   double *q0, *q1, *hb0, *hb1;
   int storm;
   or_storm st;
+  int eloss;                                /* energy loss: per-cell fx, fy */
+  double *fdx, *fdy;
+  int infil;                                /* infiltration: rate, capacity (state) */
+  double *irate, *icap;
+  int eddy;                                 /* 1 constant mu field, 2 Smagorinsky */
+  double mu_const, mu_smag;
+  double *mu, *d2[4];                       /* d2u/dx2, d2u/dy2, d2v/dx2, d2v/dy2 */
+  int nstage;
+  or_series stage[OR_MAX_BLOCKS];
+  double *sb0, *sb1;                        /* stage (water level) endpoints */
   double ox, oy;
   double start, end, step_dur, dt0, dt;
   int64_t n_acc, n_rej;
@@ -347,6 +357,8 @@ or_sim* or_sim_create(int64_t nx, int64_t ny, double dx, double dy, int nstages,
   S->hb0 = zalloc(S->nc); S->hb1 = zalloc(S->nc);
   for (int64_t i = 0; i < S->nc; ++i) { S->hb0[i] = NAN; S->hb1[i] = NAN; }
   S->nh = zalloc(S->nc); S->Sf = zalloc(S->nc);
+  S->sb0 = zalloc(S->nc); S->sb1 = zalloc(S->nc);
+  for (int64_t i = 0; i < S->nc; ++i) { S->sb0[i] = NAN; S->sb1[i] = NAN; }
   S->start = 0.0; S->end = 0.0; S->step_dur = 60.0; S->dt0 = 1.0; S->dt = 1.0;
   return S;
 }
@@ -367,6 +379,10 @@ void or_sim_destroy(or_sim* S) {
   free(S->nh); free(S->Sf);
   for (int b = 0; b < S->ndis; ++b) free_series(&S->dis[b]);
   for (int b = 0; b < S->nhead; ++b) free_series(&S->head[b]);
+  for (int b = 0; b < S->nstage; ++b) free_series(&S->stage[b]);
+  free(S->fdx); free(S->fdy); free(S->irate); free(S->icap); free(S->mu);
+  for (int k = 0; k < 4; ++k) free(S->d2[k]);
+  free(S->sb0); free(S->sb1);
   free(S);
 }
 
@@ -425,6 +441,30 @@ int or_sim_add_discharge(or_sim* S, int n, const double* t, const double* v,
 int or_sim_add_head(or_sim* S, int n, const double* t, const double* v,
                     const uint8_t* sel) {
   return add_series(S, S->head, &S->nhead, n, t, v, sel);
+}
+
+int or_sim_add_stage(or_sim* S, int n, const double* t, const double* v, const uint8_t* sel) {
+  return add_series(S, S->stage, &S->nstage, n, t, v, sel);
+}
+
+void or_sim_set_energy_loss(or_sim* S, const double* fx, const double* fy) {
+  free(S->fdx); free(S->fdy);
+  S->fdx = dupd(fx, S->nc); S->fdy = dupd(fy, S->nc);
+  S->eloss = 1;
+}
+
+void or_sim_set_infiltration(or_sim* S, const double* rate, const double* capacity) {
+  free(S->irate); free(S->icap);
+  S->irate = dupd(rate, S->nc); S->icap = dupd(capacity, S->nc);
+  S->infil = 1;
+}
+
+void or_sim_set_eddy_viscosity(or_sim* S, const double* mu, double mu_const, double mu_smag) {
+  free(S->mu);
+  S->mu = mu ? dupd(mu, S->nc) : zalloc(S->nc);
+  for (int k = 0; k < 4; ++k) { free(S->d2[k]); S->d2[k] = zalloc(S->nc); }
+  S->mu_const = mu_const; S->mu_smag = mu_smag;
+  S->eddy = mu ? 1 : 2;
 }
 
 void or_sim_set_storm(or_sim* S, const or_storm* st, double origin_x, double origin_y) {
@@ -494,6 +534,18 @@ void or_sim_start_new_step(or_sim* S, double time_now) {
           S->q0[i] += or_storm_rate(&S->st, cx, cy, time_now);
           S->q1[i] += or_storm_rate(&S->st, cx, cy, time_now + sd);
         }
+    }
+  }
+  /* StageBoundarySourceTerm.hpp:114-142: water-level endpoints */
+  if (S->nstage > 0) {
+    field_assign(S->sb0, S->nc, 0.0);
+    field_assign(S->sb1, S->nc, 0.0);
+    for (int b = 0; b < S->nstage; ++b) {
+      double a0 = ts_linterp(&S->stage[b], time_now);
+      double a1 = ts_linterp(&S->stage[b], time_now + sd);
+      const uint8_t* sel = S->stage[b].sel;
+      for (int64_t i = 0; i < S->nc; ++i)
+        if (!sel || sel[i]) { S->sb0[i] = a0; S->sb1[i] = a1; }
     }
   }
   if (S->nhead > 0) {
@@ -661,8 +713,101 @@ static void head_pass(or_sim* S, const double* h_, double* dh, double timestep,
   }
 }
 
+/* EnergyLossSourceTerm.cpp:59-88: quadratic loss with per-cell fx, fy. */
+static void eloss_pass(or_sim* S, const double* h_, const double* u_, const double* v_,
+                       double* du, double* dv, double timestep) {
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < S->nc; ++c) {
+    double h = h_[c], u = u_[c], v = v_[c];
+    if (h > 1e-6) {
+      double hyp = sqrt(u * u + v * v);
+      double dudt = -0.5 * u * S->fdx[c] * hyp;
+      double dvdt = -0.5 * v * S->fdy[c] * hyp;
+      if (fabs(dudt * timestep) > fabs(u) && sy_sign(dudt * timestep) != sy_sign(u))
+        dudt = -u / timestep;
+      if (fabs(dvdt * timestep) > fabs(v) && sy_sign(dvdt * timestep) != sy_sign(v))
+        dvdt = -v / timestep;
+      du[c] += dudt;
+      dv[c] += dvdt;
+    }
+  }
+}
+
+/* InfiltrationSourceTerm.cpp:42-64: the capacity is state and shrinks at
+ * every application (every stage, rejected timesteps included). */
+static void infil_pass(or_sim* S, const double* h_, double* dh, double timestep) {
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < S->nc; ++c) {
+    double h = h_[c];
+    double d = S->irate[c] * timestep;
+    if (d > h) d = h;
+    if (d > S->icap[c]) d = S->icap[c];
+    dh[c] -= d / timestep;
+    S->icap[c] -= d;
+  }
+}
+
+/* EddyViscositySourceTerm.hpp:216-256 and .cpp:44-98: second derivatives
+ * are Minmod3 slopes of the first-derivative fields (x of dU/dx, y of
+ * dU/dy); Smagorinsky mu from the first derivatives.  pow(a, 2) is a*a. */
+static void eddy_pass(or_sim* S, const double* h_, const double* u_, const double* v_,
+                      double* du, double* dv, double timestep) {
+  or_slope_x(S->nx, S->ny, S->dx, S->dUdx[1], S->d2[0]);
+  or_slope_y(S->nx, S->ny, S->dy, S->dUdy[1], S->d2[1], S->flags);
+  or_slope_x(S->nx, S->ny, S->dx, S->dUdx[2], S->d2[2]);
+  or_slope_y(S->nx, S->ny, S->dy, S->dUdy[2], S->d2[3], S->flags);
+  if (S->eddy == 2) {
+    const double area = S->dx * S->dy;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < S->nc; ++c) {
+      double dudx = S->dUdx[1][c], dudy = S->dUdy[1][c];
+      double dvdx = S->dUdx[2][c], dvdy = S->dUdy[2][c];
+      double t = fabs(dudy) + fabs(dvdx);
+      S->mu[c] = S->mu_const + S->mu_smag * area *
+                 sqrt(dudx * dudx + dvdy * dvdy + 0.5 * (t * t));
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < S->nc; ++c) {
+    double h = h_[c], u = u_[c], v = v_[c];
+    if (h > 1e-6) {
+      double dudt = -S->mu[c] * (S->d2[0][c] + S->d2[1][c]);
+      double dvdt = -S->mu[c] * (S->d2[2][c] + S->d2[3][c]);
+      if (fabs(dudt * timestep) > fabs(u) && sy_sign(dudt * timestep) != sy_sign(u))
+        dudt = -u / timestep;
+      if (fabs(dvdt * timestep) > fabs(v) && sy_sign(dvdt * timestep) != sy_sign(v))
+        dvdt = -v / timestep;
+      du[c] += dudt;
+      dv[c] += dvdt;
+    }
+  }
+}
+
+/* StageBoundarySourceTerm.cpp:43-68: relax the depth to water level minus
+ * bed (overwrites dh where the level is set). */
+static void stage_pass(or_sim* S, const double* h_, double* dh, double timestep,
+                       double time_now) {
+  const double step_length = S->step_dur;
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < S->nc; ++c) {
+    double z = S->zb[c];
+    double h0 = S->sb0[c] - z, h1 = S->sb1[c] - z;
+    if (!(h0 != h0) && !(h1 != h1)) {
+      double h = h_[c];
+      double dhbdt = (h1 - h0) / step_length;
+      double now = h0 + time_now * dhbdt;
+      double next = now + timestep * dhbdt;
+      double target = 0.5 * (now + next);
+      if (target <= 0.0) target = 0.0;
+      dh[c] = (target - h) / timestep;
+    }
+  }
+}
+
 /* Solver.cpp:77-114: slopes -> faces -> cell pass -> sources -> boundaries.
- * The configured order is Manning, then discharge blocks, then head blocks. */
+ * Sources and boundaries run in config order; the order restated here is
+ * Manning, energy loss, infiltration, eddy viscosity, then the discharge,
+ * head and stage blocks. */
 void or_sim_update_dudt(or_sim* S, int s, double t_sub, double dt) {
   const double* h = S->Uh[s];
   const double* u = S->Uu[s];
@@ -676,8 +821,12 @@ void or_sim_update_dudt(or_sim* S, int s, double t_sub, double dt) {
   face_pass(S, h, u, v);
   cell_pass(S, h, S->Kh[s], S->Ku[s], S->Kv[s]);
   if (S->manning) manning_pass(S, h, u, v, S->Ku[s], S->Kv[s], dt);
+  if (S->eloss) eloss_pass(S, h, u, v, S->Ku[s], S->Kv[s], dt);
+  if (S->infil) infil_pass(S, h, S->Kh[s], dt);
+  if (S->eddy) eddy_pass(S, h, u, v, S->Ku[s], S->Kv[s], dt);
   if (S->ndis > 0 || S->storm) discharge_pass(S, h, S->Kh[s], dt, t_sub);
   if (S->nhead > 0) head_pass(S, h, S->Kh[s], dt, t_sub);
+  if (S->nstage > 0) stage_pass(S, h, S->Kh[s], dt, t_sub);
 }
 
 /* RungeKutta.hpp:371-405: y_i = ((y_0 + k_0*(a dt)) + k_1*(a dt)) + ... */
@@ -869,6 +1018,11 @@ double* or_sim_array(or_sim* S, int which, int idx) {
     case 23: return S->hb1;
     case 24: return S->nh;
     case 25: return S->Sf;
+    case 26: return S->icap;
+    case 27: return S->mu;
+    case 28: case 29: case 30: case 31: return S->d2[which - 28];
+    case 32: return S->sb0;
+    case 33: return S->sb1;
     default: return NULL;
   }
 }

@@ -22,6 +22,12 @@
  *   or_td (cell pass)    lake-at-rest => dU/dt == 0 exactly; telescoping sum
  *   or_manning           closed form incl. overshoot cap
  *   or_discharge/head    closed form incl. the dry guard
+ *   energy loss          closed form incl. the overshoot cap and dry cut
+ *   infiltration         closed form: rate, depth and capacity limits, the
+ *                        capacity consumed once per application
+ *   eddy viscosity       quadratic fields: exact second derivatives; the
+ *                        Smagorinsky coefficient of linear fields
+ *   stage boundary       closed form: target = level - bed (head invariant)
  *   RK + controller      tableau KATs; Heun-2 on a linear ODE; controller
  *                        table over cn; landing rules
  *   volume               exact for constant fields
@@ -32,7 +38,11 @@
  *   - the last bits of sycl::pow in the Manning term: here x^(4/3) comes
  *     from or_pow43 (IEEE-only Newton iteration, within 7 ulp, the same
  *     bits as the HIP kernels);
- *   - smoothstep with d_shallow == d_deep (0/0); configs keep d_sh < d_dp.
+ *   - smoothstep with d_shallow == d_deep (0/0); configs keep d_sh < d_dp;
+ *   - sycl::pow(a, 2) in the Smagorinsky coefficient: restated as a * a;
+ *   - the order of source terms and boundaries, which the reference takes
+ *     from its config file: restated in one fixed order (Manning, energy
+ *     loss, infiltration, eddy viscosity; discharge, head, stage).
  */
 #ifndef MFSV_SV_ORACLE_H
 #define MFSV_SV_ORACLE_H
@@ -108,6 +118,18 @@ int or_sim_add_head(or_sim* S, int n, const double* t, const double* v,
                     const uint8_t* sel);
 void or_sim_set_storm(or_sim* S, const or_storm* st, double origin_x,
                       double origin_y);
+/* Stage boundary blocks (water-level time series on a selection). */
+int or_sim_add_stage(or_sim* S, int n, const double* t, const double* v,
+                     const uint8_t* sel);
+/* Energy loss with per-cell coefficients fx, fy (per metre). */
+void or_sim_set_energy_loss(or_sim* S, const double* fx, const double* fy);
+/* Infiltration: per-cell rate (m/s) and initial capacity (m); the capacity
+ * is state (array 26). */
+void or_sim_set_infiltration(or_sim* S, const double* rate, const double* capacity);
+/* Eddy viscosity: a per-cell constant mu, or (mu == NULL) Smagorinsky with
+ * mu = mu_const + mu_smag * dx dy * sqrt(...). */
+void or_sim_set_eddy_viscosity(or_sim* S, const double* mu, double mu_const,
+                               double mu_smag);
 void or_sim_set_time(or_sim* S, double start, double end, double step_duration,
                      double initial_dt);
 void or_sim_set_dry_depth(or_sim* S, double h_dry);
@@ -136,7 +158,9 @@ int64_t or_sim_trace_len(const or_sim* S);
 /* Internal arrays for white-box tests.  which: 0 h, 1 u, 2 v (state idx);
  * 3..5 dUdt h,u,v (idx); 6..8 dUdx h,u,v; 9..11 dUdy h,u,v; 12 zb; 13 dzbdx;
  * 14 dzbdy; 15..19 faces hflux,uflux,vflux,zflux,branch; 20 qbdy0; 21 qbdy1;
- * 22 hbdy0; 23 hbdy1; 24 mannings_n; 25 friction_slope. */
+ * 22 hbdy0; 23 hbdy1; 24 mannings_n; 25 friction_slope; 26 infiltration
+ * capacity; 27 eddy mu; 28..31 d2u/dx2, d2u/dy2, d2v/dx2, d2v/dy2; 32, 33
+ * stage endpoints. */
 double* or_sim_array(or_sim* S, int which, int idx);
 
 #ifdef __cplusplus

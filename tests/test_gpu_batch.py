@@ -80,3 +80,40 @@ def test_member_joining_a_running_batch(gpu_lib):
         for x, y in zip(c.state(), state):
             np.testing.assert_array_equal(x, y)
     b.close()
+
+
+@pytest.mark.parametrize("prec_name", ["f32", "f64"])
+def test_batch_with_further_sources_equals_alone(gpu_lib, prec_name):
+    """Members with infiltration, energy loss, Smagorinsky eddy viscosity or
+    a coded-out (NaN) bed cell share a launch with plain members: the batch
+    takes its general kernel, each member its own tile code."""
+    from mfcm_amd import mfsv
+    from mfcm_amd.batch import Batch
+    prec = mfsv.F32 if prec_name == "f32" else mfsv.F64
+    scs = _members(4, t_end=90.0)
+    rng = np.random.default_rng(5)
+    s1 = scs[1]
+    s1.infiltration = (rng.uniform(0, 1e-5, (s1.ny, s1.nx)), np.full((s1.ny, s1.nx), 2e-3))
+    s1.energy_loss = (0.01, 0.02)
+    scs[2].eddy = {"smagorinsky": (0.05, 0.5)}
+    zb = scs[3].zb.copy()
+    zb[scs[3].ny // 2, scs[3].nx // 2] = np.nan
+    scs[3].zb = zb
+    _say(prec_name, "members", [(s.nx, s.ny) for s in scs])
+    ref = [_alone(sc, prec) for sc in scs]
+    b = Batch(prec)
+    cats = []
+    for sc in scs:
+        c = mfsv.Catchment(sc, precision=prec)
+        c.run_begin()
+        b.add(c)
+        cats.append(c)
+    assert b.run() == 0
+    for c, (state, acc, rej) in zip(cats, ref):
+        st = c.stats()
+        assert (st["accepted"], st["rejected"]) == (acc, rej)
+        for x, y in zip(c.state(), state):
+            np.testing.assert_array_equal(x, y)
+    b.close()
+    for c in cats:
+        c.close()

@@ -175,3 +175,83 @@ def test_hillslope_256_matches_oracle(gpu_lib):
     vg, vo = cat.volume(), sim.volume(0)
     assert abs(vg - vo) <= 1e-10 * max(abs(vo), 1e-30)
     cat.close()
+
+
+def _with_sources(sc, seed, kinds):
+    """Switch on the further source terms / stage boundary on a scenario."""
+    rng = np.random.default_rng(seed)
+    ny, nx = sc.ny, sc.nx
+    if "eloss" in kinds:
+        sc.energy_loss = (rng.uniform(0.0, 0.05, (ny, nx)), rng.uniform(0.0, 0.05, (ny, nx)))
+    if "infil" in kinds:
+        sc.infiltration = (rng.uniform(0.0, 2e-3, (ny, nx)), rng.uniform(0.0, 5e-3, (ny, nx)))
+    if "eddy" in kinds:
+        sc.eddy = {"mu": rng.uniform(0.0, 0.3, (ny, nx))}
+    if "smag" in kinds:
+        sc.eddy = {"smagorinsky": (0.05, 0.5)}
+    if "stage" in kinds:
+        sc.stage_t = np.array([0.0, 1e6])
+        lvl = float(np.median(sc.zb)) + 0.4
+        sc.stage_v = np.array([lvl, lvl])
+        sel = np.zeros((ny, nx), bool)
+        sel[:, -1] = True
+        sc.stage_sel = sel
+    return sc
+
+
+@pytest.mark.parametrize("nx,ny,dx,flags,kinds,seed", [
+    (64, 48, 5.0, 0, ("eloss", "infil", "eddy", "stage"), 21),
+    (70, 40, 2.0, S.E7_Y_ORIENT, ("eloss", "infil", "smag", "stage"), 22),   # power-of-two spacing
+    (61, 17, 3.0, S.E7_Y_ORIENT | S.E8_DRY_VELOCITY, ("smag",), 23),
+    (13, 9, 4.0, 0, ("eddy", "infil"), 24),
+])
+def test_dudt_further_sources_match_oracle(gpu_lib, nx, ny, dx, flags, kinds, seed):
+    """Energy loss, infiltration (and the capacity it consumes), eddy
+    viscosity (per-cell and Smagorinsky) and the stage boundary, one dU/dt
+    evaluation: HIP vs oracle."""
+    from mfcm_amd.mfsv import Catchment
+    _say(f"dudt further sources nx={nx} ny={ny} dx={dx} flags={flags} {kinds}")
+    zb, h, u, v = _random_state(nx, ny, seed)
+    sc = _scenario_from_state(nx, ny, dx, dx, zb, h, u, v, flags=flags, rain=2e-5)
+    sc = _with_sources(sc, seed, kinds)
+    sim = _oracle(sc)
+    sim.start_new_step(30.0)
+    dt, t_sub = 0.37, 4.5
+    sim.update_dudt(0, t_sub, dt)
+    cat = Catchment(sc)
+    cat.start_new_step(30.0)
+    kh, ku, kv = cat.dudt(t_sub, dt)
+    for name, g in (("dhdt", kh), ("dudt", ku), ("dvdt", kv)):
+        o = sim.array(name).reshape(ny, nx)
+        err = np.abs(g - o) / (1.0 + np.abs(o))
+        _say(f"  {name}: {int(np.sum(g != o))} of {g.size} values differ in any bit")
+        assert np.all(np.isfinite(g)), name
+        assert err.max() <= 1e-12, (name, err.max(), np.unravel_index(err.argmax(), err.shape))
+    if "infil" in kinds:
+        np.testing.assert_array_equal(cat.download("infil_cap").ravel(), sim.array("infil_capacity"))
+    cat.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_run_further_sources_matches_oracle(gpu_lib, graph):
+    """A dam break with infiltration, energy loss, Smagorinsky eddy viscosity
+    and a constant stage boundary on the east column, run to 60 s: the
+    device-controlled run vs the oracle (state, counts, remaining capacity)."""
+    from mfcm_amd.mfsv import Catchment
+    sc = S.dam_break_bump(64, 48)
+    sc.t_end = 60.0
+    sc = _with_sources(sc, 31, ("eloss", "infil", "smag", "stage"))
+    _say(f"run further sources graph={graph}")
+    sim = _oracle(sc)
+    sim.solve()
+    cat = Catchment(sc)
+    cat.run_begin()
+    st = cat.run(graph=graph)
+    _compare_states(cat, sim)
+    acc, rej = sim.stats()
+    assert (st["accepted"], st["rejected"]) == (acc, rej)
+    cap_g = cat.download("infil_cap").ravel()
+    assert np.abs(cap_g - sim.array("infil_capacity")).max() <= 1e-12
+    resid = st["volume"] - st["volume0"] - st["source_volume"]
+    assert abs(resid) <= 1e-9 * max(st["volume"], 1.0), resid
+    cat.close()

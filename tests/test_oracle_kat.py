@@ -413,3 +413,132 @@ def test_e8_dry_cells_carry_no_velocity_and_e9_caps():
     assert np.all(u1[h1 <= 1e-3] == 0.0) and np.all(v1[h1 <= 1e-3] == 0.0)
     assert np.all(np.abs(u1) <= 10.0) and np.all(np.abs(v1) <= 10.0)
     assert np.any(u1 == 10.0)
+
+
+# ------------------------------------------------ further sources/boundaries
+def _flat_sim(n=5, h=1.0, u=0.0, v=0.0, dx=2.0, flags=0):
+    sim = O.Sim(n, n, dx, dx, "heun2", flags)
+    sim.set_bed(np.zeros((n, n)))
+    sim.set_state(np.full((n, n), h), np.full((n, n), u), np.full((n, n), v))
+    sim.set_time(0.0, 60.0, 60.0, 1.0)
+    return sim
+
+
+def test_energy_loss_closed_form_cap_and_dry_cut():
+    # uniform flow on a flat bed: interior flux divergence and pressure are
+    # exactly zero, so the interior dU/dt is the energy-loss term alone
+    u, v, fx, fy, dt = 0.5, -0.3, 0.02, 0.04, 0.25
+    sim = _flat_sim(u=u, v=v)
+    sim.set_energy_loss(fx, fy)
+    sim.start_new_step(0.0)
+    sim.update_dudt(0, 0.0, dt)
+    hyp = math.sqrt(u * u + v * v)
+    assert sim.array("dudt").reshape(5, 5)[2, 2] == -0.5 * u * fx * hyp
+    assert sim.array("dvdt").reshape(5, 5)[2, 2] == -0.5 * v * fy * hyp
+    # overshoot: the loss would reverse u within dt -> -u/dt
+    sim = _flat_sim(u=u, v=v)
+    sim.set_energy_loss(1e4, 0.0)
+    sim.start_new_step(0.0)
+    sim.update_dudt(0, 0.0, dt)
+    assert sim.array("dudt").reshape(5, 5)[2, 2] == -u / dt
+    # no loss in dry cells (h <= 1e-6)
+    sim = _flat_sim(h=5e-7, u=u, v=v)
+    sim.set_energy_loss(fx, fy)
+    sim.start_new_step(0.0)
+    sim.update_dudt(0, 0.0, dt)
+    assert sim.array("dudt").reshape(5, 5)[2, 2] == 0.0
+
+
+@pytest.mark.parametrize("rate,cap,h,limit", [
+    (1e-4, 0.1, 0.01, "rate"), (1.0, 0.1, 0.01, "depth"), (1.0, 0.002, 0.01, "capacity")])
+def test_infiltration_limits_and_capacity_state(rate, cap, h, limit):
+    dt = 0.5
+    sim = _flat_sim(h=h)
+    sim.set_infiltration(rate, cap)
+    sim.start_new_step(0.0)
+    sim.update_dudt(0, 0.0, dt)
+    d = {"rate": rate * dt, "depth": h, "capacity": cap}[limit]
+    assert sim.array("dhdt").reshape(5, 5)[2, 2] == -d / dt
+    assert sim.array("infil_capacity").reshape(5, 5)[2, 2] == cap - d
+    # the capacity is consumed again by every further application (every
+    # stage, rejected timesteps included): apply once more to the same state
+    sim.update_dudt(0, 0.0, dt)
+    c1 = cap - d
+    d2 = min(rate * dt, h, c1)
+    assert sim.array("infil_capacity").reshape(5, 5)[2, 2] == c1 - d2
+    assert sim.array("dhdt").reshape(5, 5)[2, 2] == -d2 / dt
+
+
+def test_stage_boundary_is_head_relative_to_bed():
+    n, dt, t = 5, 0.5, 10.0
+    z = 2.0
+    lev_t, lev_v = [0.0, 60.0, 1e6], [3.5, 4.1, 4.1]
+    sel = np.zeros((n, n), np.uint8)
+    sel[:, 0] = 1
+    a = O.Sim(n, n, 2.0, 2.0, "heun2", 0)
+    a.set_bed(np.full((n, n), z))
+    a.set_state(np.full((n, n), 1.0))
+    a.set_time(0.0, 60.0, 60.0, 1.0)
+    a.add_stage(lev_t, lev_v, sel)
+    a.start_new_step(0.0)
+    a.update_dudt(0, t, dt)
+    b = O.Sim(n, n, 2.0, 2.0, "heun2", 0)
+    b.set_bed(np.full((n, n), z))
+    b.set_state(np.full((n, n), 1.0))
+    b.set_time(0.0, 60.0, 60.0, 1.0)
+    b.add_head(lev_t, [x - z for x in lev_v], sel)
+    b.start_new_step(0.0)
+    b.update_dudt(0, t, dt)
+    np.testing.assert_array_equal(a.array("dhdt"), b.array("dhdt"))
+    # closed form at one boundary cell: target = mean of the level - bed at t, t+dt
+    h0, h1 = 3.5 - z, 4.1 - z
+    dh = (h1 - h0) / 60.0
+    now = h0 + t * dh
+    target = 0.5 * (now + (now + dt * dh))
+    assert a.array("dhdt").reshape(n, n)[2, 0] == (target - 1.0) / dt
+    assert a.array("dhdt").reshape(n, n)[2, 2] != (target - 1.0) / dt  # not selected
+
+
+@pytest.mark.parametrize("flags", [0, O.E7_Y_ORIENT])
+def test_eddy_viscosity_second_derivatives_of_quadratics(flags):
+    n, dx = 12, 1.0
+    a, b = 0.25, 0.125
+    x = np.arange(n, dtype=np.float64)[None, :] * dx + np.zeros((n, 1))
+    y = np.arange(n, dtype=np.float64)[:, None] * dx + np.zeros((1, n))
+    sim = O.Sim(n, n, dx, dx, "heun2", flags)
+    sim.set_bed(np.zeros((n, n)))
+    sim.set_state(np.ones((n, n)), a * x * x, b * y * y)
+    sim.set_time(0.0, 60.0, 60.0, 1.0)
+    sim.set_eddy_viscosity(0.7)
+    sim.start_new_step(0.0)
+    sim.update_dudt(0, 0.0, 0.1)
+    inner = (slice(3, n - 3), slice(3, n - 3))
+    # Minmod3 of the Minmod3 slope of a x^2 is exactly 2a inside (both sign
+    # flips of the reference's y operator cancel in d2/dy2)
+    assert np.all(sim.array("d2udx2").reshape(n, n)[inner] == 2 * a)
+    assert np.all(sim.array("d2vdy2").reshape(n, n)[inner] == 2 * b)
+    assert np.all(sim.array("d2udy2").reshape(n, n)[inner] == 0.0)
+    assert np.all(sim.array("d2vdx2").reshape(n, n)[inner] == 0.0)
+    assert np.all(sim.array("eddy_mu").reshape(n, n) == 0.7)
+
+
+@pytest.mark.parametrize("flags", [0, O.E7_Y_ORIENT])
+def test_smagorinsky_coefficient_of_linear_fields(flags):
+    n, dx, dy = 9, 2.0, 4.0
+    al, be = 0.5, 0.25
+    x = (np.arange(n, dtype=np.float64)[None, :] * dx) + np.zeros((n, 1))
+    y = (np.arange(n, dtype=np.float64)[:, None] * dy) + np.zeros((1, n))
+    sim = O.Sim(n, n, dx, dy, "heun2", flags)
+    sim.set_bed(np.zeros((n, n)))
+    sim.set_state(np.ones((n, n)), al * x, be * y)
+    sim.set_time(0.0, 60.0, 60.0, 1.0)
+    sim.set_eddy_viscosity(None, 0.05, 0.5)
+    sim.start_new_step(0.0)
+    sim.update_dudt(0, 0.0, 0.1)
+    # du/dx = al, dv/dy = -+be (the reference's y slope is north->south
+    # unless E7), cross terms 0: mu = c + s dx dy sqrt(al^2 + be^2)
+    want = 0.05 + 0.5 * (dx * dy) * math.sqrt(al * al + be * be + 0.5 * (0.0 * 0.0))
+    mu = sim.array("eddy_mu").reshape(n, n)
+    assert np.all(mu[1:-1, 1:-1] == want)
+    # linear fields: zero second derivatives, no eddy force
+    assert np.all(sim.array("d2udx2").reshape(n, n)[1:-1, 1:-1] == 0.0)
