@@ -408,6 +408,36 @@ def breach_section(horizon):
     return out
 
 
+def sources_section(size, steps, warmup):
+    """The headline catchment with the further source terms on: energy loss
+    (per-cell fx, fy), infiltration (per-cell rate and capacity, consumed at
+    every stage) and Smagorinsky eddy viscosity.  Algorithmic bytes per
+    cell-update: s * (15 + 2m + 4) with m = 5 static fields (bed, n, fx, fy,
+    rate) and the capacity read and written in both stages."""
+    import copy
+
+    from mfcm_amd import mfsv
+    rng = np.random.default_rng(11)
+    sc = copy.copy(make_scenario(size))
+    sc.energy_loss = (rng.uniform(0.0, 0.02, (size, size)), rng.uniform(0.0, 0.02, (size, size)))
+    sc.infiltration = (np.full((size, size), 2e-6), np.full((size, size), 0.05))
+    sc.eddy = {"smagorinsky": (0.05, 0.5)}
+    out = {"workload": f"storm catchment {size}x{size} + energy loss + infiltration + "
+                       f"Smagorinsky eddy viscosity", "flags": int(sc.flags)}
+    for name, prec, s_ in (("f32", mfsv.F32, 4), ("f64", mfsv.F64, 8)):
+        r = time_precision(sc, prec, steps, warmup, 40)
+        B = s_ * (15 + 2 * 5 + 4)
+        ach = (B / 2.0) * sc.ncells / (r["stage_ms_mean"] * 1e-3) / 1e9
+        out[name] = {"cell_updates_per_s": r["cell_updates_per_s"], "tile_rows": r["tile_rows"],
+                     "stage_us_mean": r["stage_ms_mean"] * 1e3,
+                     "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                                  "bytes_per_cell_update": B}}
+        log(f"  sources {name}: {r['cell_updates_per_s']:.4g} cell-updates/s, "
+            f"stage {r['stage_ms_mean'] * 1e3:.1f} us")
+    return out
+
+
 def cpu_baseline(sc, budget_s):
     """The CPU oracle on the first timesteps of the same workload."""
     from oracle import oracle as O
@@ -528,7 +558,8 @@ def main():
                       args.serve_budget_gb)
     cfgs = None
     if not args.no_configs:
-        cfgs = {"hillslope_256": hillslope_section(), "breach_4096": breach_section(args.breach_horizon)}
+        cfgs = {"hillslope_256": hillslope_section(), "breach_4096": breach_section(args.breach_horizon),
+                "further_sources_2048": sources_section(args.size, 400, 50)}
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -45,6 +45,7 @@ struct mfsv_ctx {
   double start = 0, end = 0, step_dur = 60, dt0 = 1;
   // device memory
   Buf U[2][3];
+  Buf Ul[2];               // FP32 form: low-order part of h of each state
   Buf acc[kMaxStages][3];  // accumulators for RK rows 1..nstages-1
   Buf zb, nfield, q0f, q1f, hb0f, hb1f, kf[3], brx, bry;
   Buf fxf, fyf, iratef, icapf, sb0f, sb1f, muf;  // further sources and the stage boundary
@@ -169,6 +170,32 @@ int get(mfsv_ctx* c, const Buf& b, double* host, int64_t n, double offset, size_
   return MFSV_OK;
 }
 
+// FP32 form: a depth is held as hi + lo (two floats, |lo| <= ulp(hi)/2);
+// upload splits the FP64 value, download returns hi + lo.
+int put_split(mfsv_ctx* c, Buf& hi, Buf& lo, const double* host, int64_t n) {
+  int rc;
+  if ((rc = alloc(c, hi, 4 * n)) || (rc = alloc(c, lo, 4 * n))) return rc;
+  std::vector<float> th(n), tl(n);
+  for (int64_t i = 0; i < n; ++i) {
+    th[i] = (float)host[i];
+    tl[i] = (float)(host[i] - (double)th[i]);
+  }
+  HIPCHK(c, hipMemcpyAsync(hi.p, th.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(lo.p, tl.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int get_split(mfsv_ctx* c, const Buf& hi, const Buf& lo, double* host, int64_t n) {
+  if (!hi.p || !lo.p) return fail(c, MFSV_ERR_STATE, "field not allocated");
+  std::vector<float> th(n), tl(n);
+  HIPCHK(c, hipMemcpyAsync(th.data(), hi.p, 4 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(tl.data(), lo.p, 4 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < n; ++i) host[i] = (double)th[i] + (double)tl[i];
+  return MFSV_OK;
+}
+
 template <typename T>
 T* tp(const Buf& b) { return static_cast<T*>(b.p); }
 
@@ -195,6 +222,7 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc)
   A.rstep = 1.0 / c->step_dur;
   A.Uh[0] = tp<T>(c->U[0][0]); A.Uu[0] = tp<T>(c->U[0][1]); A.Uv[0] = tp<T>(c->U[0][2]);
   A.Uh[1] = tp<T>(c->U[1][0]); A.Uu[1] = tp<T>(c->U[1][1]); A.Uv[1] = tp<T>(c->U[1][2]);
+  A.Uhl[0] = tp<T>(c->Ul[0]); A.Uhl[1] = tp<T>(c->Ul[1]);
   if (s > 0) {  // stage input: accumulator row s
     A.in_h = tp<T>(c->acc[s][0]); A.in_u = tp<T>(c->acc[s][1]); A.in_v = tp<T>(c->acc[s][2]);
   }
@@ -326,13 +354,14 @@ int read_ctrl(mfsv_ctx* c) {
 }
 
 template <typename T>
-__global__ void k_volume(const T* __restrict__ h, int64_t n, double* part) {
+__global__ void k_volume(const T* __restrict__ h, const T* __restrict__ hl, int64_t n, double* part) {
   __shared__ double sh[256];
   const int64_t per = (n + gridDim.x - 1) / gridDim.x;
   const int64_t b0 = blockIdx.x * per;
   const int64_t b1 = (b0 + per < n) ? b0 + per : n;
   double s = 0.0;
-  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) s += (double)h[i];
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x)
+    s += hl ? (double)h[i] + (double)hl[i] : (double)h[i];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -361,10 +390,10 @@ int compute_volume(mfsv_ctx* c, int which, double* out) {
   double* part = tp<double>(c->volp);
   if (c->prec == MFSV_F64)
     hipLaunchKernelGGL(k_volume<double>, dim3(kVolBlocks), dim3(256), 0, c->stream,
-                       tp<double>(c->U[which][0]), c->nc, part);
+                       tp<double>(c->U[which][0]), (const double*)nullptr, c->nc, part);
   else
     hipLaunchKernelGGL(k_volume<float>, dim3(kVolBlocks), dim3(256), 0, c->stream,
-                       tp<float>(c->U[which][0]), c->nc, part);
+                       tp<float>(c->U[which][0]), tp<float>(c->Ul[which]), c->nc, part);
   hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, c->stream, part, kVolBlocks, c->dx * c->dy,
                      part + kVolBlocks);
   HIPCHK(c, hipGetLastError());
@@ -430,6 +459,7 @@ int mfsv_create(const mfsv_grid* g, int precision, int device, mfsv_ctx** out) {
   const size_t fb = c->vsz * c->nc;
   for (int k = 0; k < 2 && !rc; ++k)
     for (int q = 0; q < 3 && !rc; ++q) rc = alloc(c, c->U[k][q], fb);
+  for (int k = 0; k < 2 && !rc && precision == MFSV_F32; ++k) rc = alloc(c, c->Ul[k], fb);
   for (int i = 1; i < c->nstages && !rc; ++i)
     for (int q = 0; q < 3 && !rc; ++q) rc = alloc(c, c->acc[i][q], fb);
   if (!rc) rc = alloc(c, c->zb, fb);
@@ -442,6 +472,8 @@ int mfsv_create(const mfsv_grid* g, int precision, int device, mfsv_ctx** out) {
     for (int k = 0; k < 2; ++k)
       for (int q = 0; q < 3; ++q)
         if (hipMemsetAsync(c->U[k][q].p, 0, fb, c->stream) != hipSuccess) rc = MFSV_ERR_HIP;
+    for (int k = 0; k < 2; ++k)
+      if (c->Ul[k].p && hipMemsetAsync(c->Ul[k].p, 0, fb, c->stream) != hipSuccess) rc = MFSV_ERR_HIP;
     if (hipMemsetAsync(c->zb.p, 0, fb, c->stream) != hipSuccess) rc = MFSV_ERR_HIP;
     Ctrl h{};
     h.step_dur = 60.0; h.dt = 1.0; h.cn_target = 1.0; h.nsteps = 0;
@@ -465,6 +497,7 @@ int mfsv_destroy(mfsv_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   for (auto& a : c->U) for (auto& b : a) release(b);
+  for (auto& b : c->Ul) release(b);
   for (auto& a : c->acc) for (auto& b : a) release(b);
   for (Buf* b : {&c->fxf, &c->fyf, &c->iratef, &c->icapf, &c->sb0f, &c->sb1f, &c->muf})
     release(*b);
@@ -597,7 +630,10 @@ int mfsv_upload(mfsv_ctx* c, int field, const double* host, int64_t n) {
     c->cur = c->hctrl->cur;
   }
   switch (field) {
-    case MFSV_FIELD_H: case MFSV_FIELD_U: case MFSV_FIELD_V:
+    case MFSV_FIELD_H:
+      if (c->prec == MFSV_F32) return put_split(c, c->U[c->cur][0], c->Ul[c->cur], host, n);
+      return put(c, c->U[c->cur][0], host, n, 0.0);
+    case MFSV_FIELD_U: case MFSV_FIELD_V:
       return put(c, c->U[c->cur][field], host, n, 0.0);
     case MFSV_FIELD_ZB: {
       double off = 0.0;
@@ -651,7 +687,10 @@ int mfsv_download(mfsv_ctx* c, int field, double* host, int64_t n) {
     c->cur = c->hctrl->cur;
   }
   switch (field) {
-    case MFSV_FIELD_H: case MFSV_FIELD_U: case MFSV_FIELD_V:
+    case MFSV_FIELD_H:
+      if (c->prec == MFSV_F32) return get_split(c, c->U[c->cur][0], c->Ul[c->cur], host, n);
+      return get(c, c->U[c->cur][0], host, n, 0.0, c->vsz);
+    case MFSV_FIELD_U: case MFSV_FIELD_V:
       return get(c, c->U[c->cur][field], host, n, 0.0, c->vsz);
     case MFSV_FIELD_ZB: return get(c, c->zb, host, n, c->zoff, c->vsz);
     case MFSV_FIELD_MANNING_N: return get(c, c->nfield, host, n, 0.0, c->vsz);

@@ -82,6 +82,7 @@ struct StageArgs {
   // inputs
   const T* in_h; const T* in_u; const T* in_v;   // stage input; null = U0
   T* Uh[2]; T* Uu[2]; T* Uv[2];
+  T* Uhl[2];                // FP32 form: low-order part of h of each state
   const T* zb;
   const T* nfield;
   const T* q0f; const T* q1f; const T* hb0f; const T* hb1f;
@@ -329,12 +330,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   struct Extra {
     T n;
     T bh[NB], bu[NB], bv[NB];
+    T lo;  // FP32 form: low-order depth of the final row's base
   };
+  T* const hl_in = (sizeof(T) == 4 && !KOUT) ? (A.stage == 0 ? (cur ? A.Uhl[1] : A.Uhl[0])
+                                                              : (cur ? A.Uhl[0] : A.Uhl[1]))
+                                             : nullptr;
   const bool base_ld = A.stage > 0;
   auto load_extra = [&](int r) {
     Extra e;
     const uint32_t o = roff(r);
     e.n = has_n ? ld(nf, o) : A.n_sh;
+    e.lo = T(0);
+    if constexpr (sizeof(T) == 4 && !KOUT) e.lo = ld(hl_in, o);
 #pragma unroll
     for (int j = 0; j < NB; ++j) { e.bh[j] = T(0); e.bu[j] = T(0); e.bv[j] = T(0); }
     if (base_ld) {
@@ -685,12 +692,27 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         }
       } else {
         // RungeKutta.hpp:377-388: y_i = y_i + k_s * (a dt)
-        T oh = h, ou = u, ov = v;
+        T oh = h, ou = u, ov = v, ol = T(0);
+        // FP32 form: the final row's depth is kept as hi + lo; its increment
+        // (with the product's rounding error) and the running low part are
+        // added to hi with an error-free TwoSum, so the FP64 volume sum of
+        // hi + lo tracks the accumulated rates without FP32 rounding drift
+        const T lo_in = X0.lo;
 #pragma unroll
         for (int j = 0; j < NACC; ++j) {
           T bh = h, bu = u, bv = v;
           if (base_ld) { bh = X0.bh[j]; bu = X0.bu[j]; bv = X0.bv[j]; }
-          T nh = bh + dhdt * sc[j];
+          T nh, nlo = T(0);
+          if (sizeof(T) == 4 && j == NACC - 1) {
+            const T p = dhdt * sc[j];
+            const T pe = fma_t(dhdt, sc[j], -p);   // exact: dhdt*sc == p + pe
+            const T b = (pe + lo_in) + p;
+            nh = bh + b;                              // TwoSum(bh, b)
+            const T bb = nh - bh;
+            nlo = (bh - (nh - bb)) + (b - bb);
+          } else {
+            nh = bh + dhdt * sc[j];
+          }
           T nu = bu + dudt * sc[j];
           T nv = bv + dvdt * sc[j];
           if (j == 0) {  // row s+1 is complete after this stage
@@ -703,12 +725,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
               nu = fmin_t(fmax_t(nu, -A.u_max), A.u_max);
               nv = fmin_t(fmax_t(nv, -A.u_max), A.u_max);
             }
-            oh = nh; ou = nu; ov = nv;
+            oh = nh; ou = nu; ov = nv; ol = nlo;
           }
           if (own) {
             st(oh_[j], wo, nh);
             st(ou_[j], wo, nu);
             st(ov_[j], wo, nv);
+            if constexpr (sizeof(T) == 4) {
+              if (j == NACC - 1) st(cur ? A.Uhl[0] : A.Uhl[1], wo, nlo);
+            }
           }
         }
         if (A.last) {
@@ -716,7 +741,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           const T cc = fsqrt_t(G * fmax_t(oh, T(0)));
           const T q = dv<QUICK>(fabs_t(ou) + cc, dx, rdx) + dv<QUICK>(fabs_t(ov) + cc, dy, rdy);
           qmax = (own && q > qmax) ? q : qmax;
-          vol += own ? double(oh) : 0.0;
+          vol += own ? double(oh) + double(ol) : 0.0;
         }
       }
     }

@@ -66,9 +66,11 @@ def poisson_stream(rate_hz, duration_s, seed=0, side_lo=256, side_hi=2048,
 
 def footprint_bytes(nx, ny, precision):
     """Device bytes of a heun2 scenario: two states and one accumulator row
-    (3 fields each), bed and Manning n, plus partial sums."""
+    (3 fields each), bed and Manning n (plus, in the FP32 form, the
+    low-order depth of both states), plus partial sums."""
     s = 4 if precision == F32 else 8
-    return int(11 * nx * ny * s * 1.02) + (1 << 20)
+    fields = 13 if precision == F32 else 11
+    return int(fields * nx * ny * s * 1.02) + (1 << 20)
 
 
 class FloodServer:
