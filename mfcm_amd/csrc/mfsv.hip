@@ -268,13 +268,17 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc)
 template <typename T, bool QUICK, bool NANBED, bool EXT>
 void launch_q(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
   dim3 grid((unsigned)c->nblocks), block(kWave * kWavesPerBlock);
-  switch (kout ? 0 : nacc) {
-    case 0: hipLaunchKernelGGL((k_stage<T, 0, true, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
-    case 1: hipLaunchKernelGGL((k_stage<T, 1, false, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
-    case 2: hipLaunchKernelGGL((k_stage<T, 2, false, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
-    case 3: hipLaunchKernelGGL((k_stage<T, 3, false, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
-    default: hipLaunchKernelGGL((k_stage<T, 4, false, QUICK, NANBED, EXT>), grid, block, 0, c->stream, A); break;
+  const bool base = A.stage > 0;
+#define MFSV_LAUNCH(N, K, B) \
+  hipLaunchKernelGGL((k_stage<T, N, K, QUICK, NANBED, EXT, B>), grid, block, 0, c->stream, A)
+  if (kout) { MFSV_LAUNCH(0, true, false); return; }
+  switch (nacc) {
+    case 1: if (base) MFSV_LAUNCH(1, false, true); else MFSV_LAUNCH(1, false, false); break;
+    case 2: if (base) MFSV_LAUNCH(2, false, true); else MFSV_LAUNCH(2, false, false); break;
+    case 3: if (base) MFSV_LAUNCH(3, false, true); else MFSV_LAUNCH(3, false, false); break;
+    default: MFSV_LAUNCH(4, false, false); break;  // four rows: only the first of four stages
   }
+#undef MFSV_LAUNCH
 }
 
 // Further source terms or the stage boundary switched on?

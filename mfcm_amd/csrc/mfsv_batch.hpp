@@ -155,12 +155,16 @@ void launch_batch_stage(mfsv_batch* b, int s) {
   const StageArgs<T>* m = static_cast<const StageArgs<T>*>(b->margs[s].p);
   const int2* t = static_cast<const int2*>(b->tiles.p);
   dim3 grid((unsigned)b->nblocks), block(kWave * kWavesPerBlock);
+  const bool base = s > 0;
+#define MFSV_BLAUNCH(N, B) \
+  hipLaunchKernelGGL((k_stage_batch<T, N, QUICK, GENERAL, B>), grid, block, 0, b->stream, m, t, cnt)
   switch (b->nstages - s) {
-    case 1: hipLaunchKernelGGL((k_stage_batch<T, 1, QUICK, GENERAL>), grid, block, 0, b->stream, m, t, cnt); break;
-    case 2: hipLaunchKernelGGL((k_stage_batch<T, 2, QUICK, GENERAL>), grid, block, 0, b->stream, m, t, cnt); break;
-    case 3: hipLaunchKernelGGL((k_stage_batch<T, 3, QUICK, GENERAL>), grid, block, 0, b->stream, m, t, cnt); break;
-    default: hipLaunchKernelGGL((k_stage_batch<T, 4, QUICK, GENERAL>), grid, block, 0, b->stream, m, t, cnt); break;
+    case 1: if (base) MFSV_BLAUNCH(1, true); else MFSV_BLAUNCH(1, false); break;
+    case 2: if (base) MFSV_BLAUNCH(2, true); else MFSV_BLAUNCH(2, false); break;
+    case 3: if (base) MFSV_BLAUNCH(3, true); else MFSV_BLAUNCH(3, false); break;
+    default: MFSV_BLAUNCH(4, false); break;
   }
+#undef MFSV_BLAUNCH
 }
 
 template <typename T, bool QUICK>

@@ -221,7 +221,9 @@ __device__ __forceinline__ Side<T> outflow_ghost(const Side<T>& c) {
 // (and with coded-out cells, NANBED) slopes are exact quotients.
 // EXT: the further source terms (energy loss, infiltration, eddy viscosity)
 // and the stage boundary are compiled in (each still switched at run time).
-template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT>
+// BASE: a later stage (s > 0), whose accumulator rows are read back as the
+// bases of this stage's updates; the first stage updates from its input.
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE>
 __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile, const int lane) {
   static_assert(!(QUICK && NANBED), "coded-out cells take the quotient path");
   constexpr bool kRel = sizeof(T) == 4;
@@ -296,8 +298,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   const bool storm = (A.flags & 0x08) != 0;
   const T dtT = T(dt);
   const T rdt = sizeof(T) == 8 ? T(1.0 / dt) : T(1.0f / float(dt));
-  // E4 storm factors: this lane's column at the step's two endpoint times,
-  // and (in lane k) row y0 + k, broadcast to all lanes row by row.
+  // E4 storm factors: this lane's column at the step's two endpoint times
+  // (registers), and the factors of rows y0 .. y0+63 (computed by lane k for
+  // row y0 + k, kept in this wave's LDS slot and read back row by row as a
+  // broadcast, so they hold no registers across the march).
+  // The FP32 form has registers to spare and keeps the row factors in lane
+  // registers, read with v_readlane.
+  constexpr bool kStormLds = sizeof(T) == 8;
+  __shared__ T storm_rows[kWavesPerBlock][2][kWave];
+  const int wv = (int)(threadIdx.x >> 6);
   T ex0 = T(0), ex1 = T(0), eyA0 = T(0), eyA1 = T(0);
   if (storm) {
     const Storm& st = A.storm;
@@ -308,6 +317,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     const double yc = A.oy + (double(y0 + lane) + 0.5) * A.dyd;
     eyA0 = storm_axis<T>(st, yc, st.cy0 + st.vy * gstep);
     eyA1 = storm_axis<T>(st, yc, st.cy0 + st.vy * t1);
+    if constexpr (kStormLds) {
+      storm_rows[wv][0][lane] = eyA0;
+      storm_rows[wv][1][lane] = eyA1;
+      // the wave's own writes, visible to all its lanes before the reads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
 
   auto load = [&](int r) {
@@ -335,7 +352,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   T* const hl_in = (sizeof(T) == 4 && !KOUT) ? (A.stage == 0 ? (cur ? A.Uhl[1] : A.Uhl[0])
                                                               : (cur ? A.Uhl[0] : A.Uhl[1]))
                                              : nullptr;
-  const bool base_ld = A.stage > 0;
+  constexpr bool base_ld = BASE;
   auto load_extra = [&](int r) {
     Extra e;
     const uint32_t o = roff(r);
@@ -344,7 +361,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     if constexpr (sizeof(T) == 4 && !KOUT) e.lo = ld(hl_in, o);
 #pragma unroll
     for (int j = 0; j < NB; ++j) { e.bh[j] = T(0); e.bu[j] = T(0); e.bv[j] = T(0); }
-    if (base_ld) {
+    if constexpr (base_ld) {
 #pragma unroll
       for (int j = 0; j < NACC; ++j) {
         e.bh[j] = ld(oh_[j], o); e.bu[j] = ld(ou_[j], o); e.bv[j] = ld(ov_[j], o);
@@ -393,6 +410,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       qnh = qy(R2.h - R1.h); qnu = qy(R2.u - R1.u); qnv = qy(R2.v - R1.v); qnz = qy(R2.z - R1.z);
       // south -> north slope; the reference's north -> south slope is its
       // exact negation (Minmod3 commutes with negation)
+      // south differences: the previous row's north differences (carried;
+      // recomputed, bit-identical, in the FP64 difference form, which keeps
+      // eight fewer registers live across the row)
+      if constexpr (QUICK && sizeof(T) == 8) {
+        qsh = R1.h - R0.h; qsu = R1.u - R0.u; qsv = R1.v - R0.v; qsz = R1.z - R0.z;
+      }
       const T yh = minmod_q(qsh, qnh, qy(R2.h - R0.h), hasS, hasN);
       const T yu = minmod_q(qsu, qnu, qy(R2.u - R0.u), hasS, hasN);
       const T yv = minmod_q(qsv, qnv, qy(R2.v - R0.v), hasS, hasN);
@@ -482,8 +505,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       if (storm) {
         const int k = r - y0;
         if (k < kWave) {
-          ey0 = lane_value(eyA0, k);
-          ey1 = lane_value(eyA1, k);
+          if constexpr (kStormLds) {
+            ey0 = storm_rows[wv][0][k];
+            ey1 = storm_rows[wv][1][k];
+          } else {
+            ey0 = lane_value(eyA0, k);
+            ey1 = lane_value(eyA1, k);
+          }
         } else {
           const Storm& st = A.storm;
           const double yc = A.oy + (double(r) + 0.5) * A.dyd;
@@ -701,7 +729,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
 #pragma unroll
         for (int j = 0; j < NACC; ++j) {
           T bh = h, bu = u, bv = v;
-          if (base_ld) { bh = X0.bh[j]; bu = X0.bu[j]; bv = X0.bv[j]; }
+          if constexpr (base_ld) { bh = X0.bh[j]; bu = X0.bu[j]; bv = X0.bv[j]; }
           T nh, nlo = T(0);
           if (sizeof(T) == 4 && j == NACC - 1) {
             const T p = dhdt * sc[j];
@@ -746,7 +774,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       }
     }
     R0 = R1; R1 = R2; R2 = Rn;
-    qsh = qnh; qsu = qnu; qsv = qnv; qsz = qnz;
+    if constexpr (!(QUICK && sizeof(T) == 8)) { qsh = qnh; qsu = qnu; qsv = qnv; qsz = qnz; }
     if constexpr (EXT) { pp_uy = p_uy; pp_vy = p_vy; p_ux = sux; p_vx = svx; }
     p_hx = shx; p_zx = szx; p_hy = shy; p_uy = suy; p_vy = svy; p_zy = szy;
     Fs0 = Fs1;
@@ -774,14 +802,22 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
 }
 
 // One catchment: one wavefront per tile.
-template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArgs<T> A) {
+// Occupancy: the FP64 difference-form kernel is held to 3 waves per SIMD
+// (<= 168 registers, at most a few bytes of scratch); the other FP64
+// variants keep 2 (they would spill), FP32 runs at 4.
+template <typename T, bool QUICK, bool NANBED, bool EXT>
+constexpr int stage_waves() {
+  return sizeof(T) == 4 ? 4 : ((QUICK && !NANBED && !EXT) ? 3 : 2);
+}
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE>
+__global__ __launch_bounds__(kWave * kWavesPerBlock)
+__attribute__((amdgpu_waves_per_eu(stage_waves<T, QUICK, NANBED, EXT>()))) void k_stage(const StageArgs<T> A) {
   const int lane = threadIdx.x & (kWave - 1);
   // wave-uniform: everything derived from the tile index stays scalar
   const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
                                                   (int)(threadIdx.x >> 6));
   if (tile >= A.ntiles) return;
-  stage_tile<T, NACC, KOUT, QUICK, NANBED, EXT>(A, tile, lane);
+  stage_tile<T, NACC, KOUT, QUICK, NANBED, EXT, BASE>(A, tile, lane);
 }
 
 // Many catchments in one launch: a persistent grid walks a tile table of
@@ -792,7 +828,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage(const StageArg
 // does not depend on which other catchments share the launch.  The table
 // length is read from device memory so that captured graphs survive changes
 // of membership.
-template <typename T, int NACC, bool QUICK, bool GENERAL>
+template <typename T, int NACC, bool QUICK, bool GENERAL, bool BASE>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
     const StageArgs<T>* __restrict__ margs, const int2* __restrict__ tiles,
     const int* __restrict__ ntiles_p) {
@@ -808,15 +844,15 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
       const StageArgs<T>& a = margs[m];
       const bool ext = (a.eloss | a.infil | a.stageb | a.eddy) != 0;
       if (a.nanbed) {
-        stage_tile<T, NACC, false, false, true, true>(a, t, lane);
+        stage_tile<T, NACC, false, false, true, true, BASE>(a, t, lane);
         continue;
       }
       if (ext) {
-        stage_tile<T, NACC, false, QUICK, false, true>(a, t, lane);
+        stage_tile<T, NACC, false, QUICK, false, true, BASE>(a, t, lane);
         continue;
       }
     }
-    stage_tile<T, NACC, false, QUICK, false, false>(margs[m], t, lane);
+    stage_tile<T, NACC, false, QUICK, false, false, BASE>(margs[m], t, lane);
   }
 }
 
