@@ -71,7 +71,26 @@ extern "C" {
 #define MFSV_FIELD_SBDY1 19    /* stage boundary: water level at step end  */
 #define MFSV_FIELD_EDDY_MU 20  /* eddy viscosity coefficient (constant
                                   model)                                 */
-#define MFSV_FIELD_COUNT 21
+/* Per-cell Manning parameters (ManningRoughnessSourceTerm.hpp:129-140, the
+ * fields n_shallow, n_deep, d_shallow, d_deep).  When all three are uploaded
+ * MANNING_N is n_shallow (it is required then). */
+#define MFSV_FIELD_N_DEEP 21
+#define MFSV_FIELD_D_SHALLOW 22
+#define MFSV_FIELD_D_DEEP 23
+/* Output fields filled by mfsv_diagnose (download only).  Face fields hold
+ * (nx+1)*ny x-faces then nx*(ny+1) y-faces in the reference's face order
+ * (Mesh/Cartesian2DMesh.hpp:150-164): the fluxes of Fluxes.hpp:81-97. */
+#define MFSV_FIELD_FLUX_H 24
+#define MFSV_FIELD_FLUX_U 25
+#define MFSV_FIELD_FLUX_V 26
+#define MFSV_FIELD_FLUX_Z 27
+#define MFSV_FIELD_MANNINGS_N 28    /* n(h) of the Manning term           */
+#define MFSV_FIELD_FRICTION_SLOPE 29 /* S_f (0 where h <= 1e-6)            */
+#define MFSV_FIELD_DZDX_BED 30      /* Minmod3 bed slopes (Constants.cpp)  */
+#define MFSV_FIELD_DZDY_BED 31
+/* z_bed + h of U0 at the last mfsv_update_stage_field (Solver.hpp:90-98). */
+#define MFSV_FIELD_STAGE 32
+#define MFSV_FIELD_COUNT 33
 
 typedef struct mfsv_ctx mfsv_ctx;
 
@@ -181,6 +200,21 @@ int mfsv_download(mfsv_ctx* ctx, int field, double* host, int64_t count);
  * SaintVenantSolver::update_dUdt (SaintVenant/Solver.cpp:77-114).  With
  * want_branches != 0 also fills BRANCH_X/BRANCH_Y (FluxKernel.cpp:291-293). */
 int mfsv_dudt(mfsv_ctx* ctx, double t_sub, double dt, int want_branches);
+/* Output-only fields of the reference's solver (the fields that
+ * MultiFieldOutputFile and the check files write, Solver.hpp:124-161):
+ * one derivative evaluation without side effects (the infiltration capacity
+ * is not consumed) that fills the selected output fields.
+ * source 0: the current state U0 at stage time t_sub with timestep dt.
+ * source 1: the input of the last RK stage of the last timestep -- the state
+ *   the reference's last update_dUdt saw, whose fluxes, Manning n and
+ *   friction slope it holds at output time (t_sub and dt are then ignored).
+ * what: MFSV_DIAG_* bits. */
+#define MFSV_DIAG_FLUXES 1     /* FLUX_H, FLUX_U, FLUX_V, FLUX_Z           */
+#define MFSV_DIAG_MANNING 2    /* MANNINGS_N, FRICTION_SLOPE (Manning on)   */
+#define MFSV_DIAG_BED_SLOPES 4 /* DZDX_BED, DZDY_BED                        */
+int mfsv_diagnose(mfsv_ctx* ctx, int source, double t_sub, double dt, int what);
+/* end_of_step (Solver.hpp:90-98): STAGE = z_bed + h of U0. */
+int mfsv_update_stage_field(mfsv_ctx* ctx);
 /* One whole timestep from U0 at local time t_local with timestep dt: every
  * RK stage (RungeKutta.cpp:143-153, update_k + update_y) fused per stage, and
  * the control number of the result (State.cpp:199-229) returned in *cn.

@@ -49,6 +49,8 @@ struct mfsv_ctx {
   Buf acc[kMaxStages][3];  // accumulators for RK rows 1..nstages-1
   Buf zb, nfield, q0f, q1f, hb0f, hb1f, kf[3], brx, bry;
   Buf fxf, fyf, iratef, icapf, sb0f, sb1f, muf;  // further sources and the stage boundary
+  Buf ndpf, dshf, ddpf;    // per-cell Manning n_deep, d_shallow, d_deep
+  Buf dgf[4], dgnh, dgsf, dgzx, dgzy, stagef;  // output-only fields
   Buf ctrl, part_vol, part_rate, forcing, volp;
   int64_t nforcing = 0;
   Ctrl* hctrl = nullptr;  // pinned mirror
@@ -58,6 +60,7 @@ struct mfsv_ctx {
   bool nfield_set = false;
   bool nanbed = false;  // the bed has coded-out (NaN) cells
   bool begun = false;   // mfsv_run_begin has initialised the control block
+  bool ran = false;     // a timestep has run (its stage inputs are held)
   mfsv_batch* batch = nullptr;  // the batch this context is a member of
   uint64_t gen = 0;     // bumped by every configuration change
   hipGraphExec_t gexec = nullptr;
@@ -238,6 +241,8 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc)
   A.irate = tp<T>(c->iratef); A.icap = tp<T>(c->icapf);
   A.sb0 = tp<T>(c->sb0f); A.sb1 = tp<T>(c->sb1f);
   A.mu = tp<T>(c->muf);
+  A.mfull = (c->ndpf.p && c->dshf.p && c->ddpf.p) ? 1 : 0;
+  A.ndp_f = tp<T>(c->ndpf); A.dsh_f = tp<T>(c->dshf); A.ddp_f = tp<T>(c->ddpf);
   A.n_sh = T(c->P.n_shallow); A.n_dp = T(c->P.n_deep);
   A.d_sh = T(c->P.d_shallow); A.d_dp = T(c->P.d_deep);
   A.h_dry = T(c->P.h_dry); A.u_max = T(c->P.u_max);
@@ -283,7 +288,8 @@ void launch_q(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
 
 // Further source terms or the stage boundary switched on?
 bool has_ext(const mfsv_ctx* c) {
-  return c->P.energy_loss || c->P.infiltration || c->P.stage || c->P.eddy;
+  return c->P.energy_loss || c->P.infiltration || c->P.stage || c->P.eddy ||
+         (c->P.manning && c->ndpf.p && c->dshf.p && c->ddpf.p);
 }
 
 template <typename T, bool QUICK, bool NANBED>
@@ -313,6 +319,41 @@ void launch_typed(mfsv_ctx* c, int s, bool kout, bool branches) {
     if (pow2(c->dx) && pow2(c->dy)) launch_x<T, true, false>(c, A, nacc, kout);
     else launch_x<T, false, false>(c, A, nacc, kout);
   }
+}
+
+// One side-effect-free derivative evaluation of stage s's input (or of the
+// previous U0 when prev_u0) that fills the selected output fields.
+template <typename T>
+void launch_diag_typed(mfsv_ctx* c, int s, bool prev_u0, int what) {
+  int nacc = 0;
+  StageArgs<T> A = stage_args<T>(c, s, true, false, &nacc);
+  if (prev_u0) {
+    const int o = c->cur ^ 1;
+    A.in_h = tp<T>(c->U[o][0]); A.in_u = tp<T>(c->U[o][1]); A.in_v = tp<T>(c->U[o][2]);
+  }
+  A.diag = 1;
+  if (what & MFSV_DIAG_FLUXES) {
+    A.dg_fh = tp<T>(c->dgf[0]); A.dg_fu = tp<T>(c->dgf[1]);
+    A.dg_fv = tp<T>(c->dgf[2]); A.dg_fz = tp<T>(c->dgf[3]);
+  }
+  if ((what & MFSV_DIAG_MANNING) && c->P.manning) { A.dg_nh = tp<T>(c->dgnh); A.dg_sf = tp<T>(c->dgsf); }
+  if (what & MFSV_DIAG_BED_SLOPES) { A.dg_zx = tp<T>(c->dgzx); A.dg_zy = tp<T>(c->dgzy); }
+  if (c->nanbed) {
+    launch_x<T, false, true>(c, A, 0, true);
+  } else if constexpr (sizeof(T) == 4) {
+    launch_x<T, true, false>(c, A, 0, true);
+  } else {
+    if (pow2(c->dx) && pow2(c->dy)) launch_x<T, true, false>(c, A, 0, true);
+    else launch_x<T, false, false>(c, A, 0, true);
+  }
+}
+
+template <typename T>
+__global__ void k_stage_field(const T* __restrict__ zb, const T* __restrict__ h, int64_t n,
+                              T* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = zb[i] + h[i];
 }
 
 int launch_stage(mfsv_ctx* c, int s, bool kout, bool branches) {
@@ -346,6 +387,7 @@ int launch_ctrl(mfsv_ctx* c, int mode) {
 
 int launch_timestep(mfsv_ctx* c, int mode) {
   int rc;
+  c->ran = true;
   for (int s = 0; s < c->nstages; ++s)
     if ((rc = launch_stage(c, s, false, false))) return rc;
   return launch_ctrl(c, mode);
@@ -503,7 +545,9 @@ int mfsv_destroy(mfsv_ctx* c) {
   for (auto& a : c->U) for (auto& b : a) release(b);
   for (auto& b : c->Ul) release(b);
   for (auto& a : c->acc) for (auto& b : a) release(b);
-  for (Buf* b : {&c->fxf, &c->fyf, &c->iratef, &c->icapf, &c->sb0f, &c->sb1f, &c->muf})
+  for (Buf* b : {&c->fxf, &c->fyf, &c->iratef, &c->icapf, &c->sb0f, &c->sb1f, &c->muf,
+                 &c->ndpf, &c->dshf, &c->ddpf, &c->dgf[0], &c->dgf[1], &c->dgf[2], &c->dgf[3],
+                 &c->dgnh, &c->dgsf, &c->dgzx, &c->dgzy, &c->stagef})
     release(*b);
   for (Buf* b : {&c->zb, &c->nfield, &c->q0f, &c->q1f, &c->hb0f, &c->hb1f, &c->kf[0], &c->kf[1],
                  &c->kf[2], &c->brx, &c->bry, &c->ctrl, &c->part_vol, &c->part_rate, &c->forcing,
@@ -670,6 +714,9 @@ int mfsv_upload(mfsv_ctx* c, int field, const double* host, int64_t n) {
     case MFSV_FIELD_SBDY0: return put(c, c->sb0f, host, n, c->zoff);
     case MFSV_FIELD_SBDY1: return put(c, c->sb1f, host, n, c->zoff);
     case MFSV_FIELD_EDDY_MU: return put(c, c->muf, host, n, 0.0);
+    case MFSV_FIELD_N_DEEP: drop_graph(c); return put(c, c->ndpf, host, n, 0.0);
+    case MFSV_FIELD_D_SHALLOW: drop_graph(c); return put(c, c->dshf, host, n, 0.0);
+    case MFSV_FIELD_D_DEEP: drop_graph(c); return put(c, c->ddpf, host, n, 0.0);
     default: return fail(c, MFSV_ERR_ARG, "field is not uploadable");
   }
 }
@@ -683,6 +730,10 @@ int mfsv_download(mfsv_ctx* c, int field, double* host, int64_t n) {
     if (n != want) return fail(c, MFSV_ERR_ARG, "branch field count mismatch");
     const Buf& b = field == MFSV_FIELD_BRANCH_X ? c->brx : c->bry;
     return get(c, b, host, n, 0.0, 4);
+  }
+  if (field >= MFSV_FIELD_FLUX_H && field <= MFSV_FIELD_FLUX_Z) {
+    if (n != nbx + nby) return fail(c, MFSV_ERR_ARG, "face field count must be (nx+1)*ny + nx*(ny+1)");
+    return get(c, c->dgf[field - MFSV_FIELD_FLUX_H], host, n, 0.0, c->vsz);
   }
   if (n != c->nc) return fail(c, MFSV_ERR_ARG, "count must equal nx*ny");
   if (field <= MFSV_FIELD_V) {
@@ -711,6 +762,14 @@ int mfsv_download(mfsv_ctx* c, int field, double* host, int64_t n) {
     case MFSV_FIELD_EDDY_MU: return get(c, c->muf, host, n, 0.0, c->vsz);
     case MFSV_FIELD_DHDT: case MFSV_FIELD_DUDT: case MFSV_FIELD_DVDT:
       return get(c, c->kf[field - MFSV_FIELD_DHDT], host, n, 0.0, c->vsz);
+    case MFSV_FIELD_N_DEEP: return get(c, c->ndpf, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_D_SHALLOW: return get(c, c->dshf, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_D_DEEP: return get(c, c->ddpf, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_MANNINGS_N: return get(c, c->dgnh, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_FRICTION_SLOPE: return get(c, c->dgsf, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_DZDX_BED: return get(c, c->dgzx, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_DZDY_BED: return get(c, c->dgzy, host, n, 0.0, c->vsz);
+    case MFSV_FIELD_STAGE: return get(c, c->stagef, host, n, c->zoff, c->vsz);
     default: return MFSV_ERR_ARG;
   }
 }
@@ -728,6 +787,11 @@ static int check_inputs(mfsv_ctx* c) {
     return fail(c, MFSV_ERR_STATE, "constant eddy viscosity needs EDDY_MU");
   if (c->P.head == 2 && (!c->hb0f.p || !c->hb1f.p))
     return fail(c, MFSV_ERR_STATE, "per-cell head needs HBDY0 and HBDY1");
+  const int mset = (c->ndpf.p ? 1 : 0) + (c->dshf.p ? 1 : 0) + (c->ddpf.p ? 1 : 0);
+  if (mset != 0 && mset != 3)
+    return fail(c, MFSV_ERR_STATE, "per-cell Manning needs N_DEEP, D_SHALLOW and D_DEEP together");
+  if (mset == 3 && !c->nfield_set)
+    return fail(c, MFSV_ERR_STATE, "per-cell Manning needs MANNING_N (the shallow n)");
   return MFSV_OK;
 }
 
@@ -746,6 +810,64 @@ int mfsv_dudt(mfsv_ctx* c, double t_sub, double dt, int want_branches) {
   // stage 0 has c = 0, so t_local = t_sub gives the stage time t_sub
   if ((rc = set_ctrl(c, t_sub, dt, 0, 1))) return rc;
   if ((rc = launch_stage(c, 0, true, want_branches != 0))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int mfsv_diagnose(mfsv_ctx* c, int source, double t_sub, double dt, int what) {
+  if (!c || (source != 0 && source != 1) || what < 0 || what > 7) return MFSV_ERR_ARG;
+  if (source == 0 && !(dt > 0)) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  if (source == 1 && !c->ran)
+    return fail(c, MFSV_ERR_STATE, "no timestep has run: there is no last stage input");
+  int rc;
+  if ((rc = check_inputs(c))) return rc;
+  if ((rc = read_ctrl(c))) return rc;
+  c->cur = c->hctrl->cur;
+  const double keep_t = c->hctrl->t_local, keep_dt = c->hctrl->dt;
+  const int64_t nf = (c->nx + 1) * c->ny + c->nx * (c->ny + 1);
+  for (int q = 0; q < 3; ++q)
+    if ((rc = alloc(c, c->kf[q], c->vsz * c->nc))) return rc;
+  if (what & MFSV_DIAG_FLUXES)
+    for (int q = 0; q < 4; ++q)
+      if ((rc = alloc(c, c->dgf[q], c->vsz * nf))) return rc;
+  if ((what & MFSV_DIAG_MANNING) && c->P.manning)
+    if ((rc = alloc(c, c->dgnh, c->vsz * c->nc)) || (rc = alloc(c, c->dgsf, c->vsz * c->nc))) return rc;
+  if (what & MFSV_DIAG_BED_SLOPES)
+    if ((rc = alloc(c, c->dgzx, c->vsz * c->nc)) || (rc = alloc(c, c->dgzy, c->vsz * c->nc))) return rc;
+  // source 1: the last stage read accumulator row n-1 (row 0 = the U0 that
+  // the accept replaced); its fluxes, n and S_f depend on the state only
+  const int s = source == 1 ? c->nstages - 1 : 0;
+  const bool prev = source == 1 && s == 0;
+  if ((rc = set_ctrl(c, source == 1 ? 0.0 : t_sub, source == 1 ? 1.0 : dt, 0, 1))) return rc;
+  if (c->prec == MFSV_F64) launch_diag_typed<double>(c, s, prev, what);
+  else launch_diag_typed<float>(c, s, prev, what);
+  HIPCHK(c, hipGetLastError());
+  // leave the control block's time as it was (a device-controlled run may
+  // continue after this call)
+  if ((rc = set_ctrl(c, keep_t, keep_dt, 0, 1))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int mfsv_update_stage_field(mfsv_ctx* c) {
+  if (!c) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  if (!c->zb.p || !c->U[0][0].p) return fail(c, MFSV_ERR_STATE, "bed and state must be uploaded");
+  int rc;
+  if ((rc = read_ctrl(c))) return rc;
+  c->cur = c->hctrl->cur;
+  if ((rc = alloc(c, c->stagef, c->vsz * c->nc))) return rc;
+  const unsigned blocks = (unsigned)std::min<int64_t>((c->nc + 255) / 256, 4096);
+  if (c->prec == MFSV_F64)
+    hipLaunchKernelGGL(k_stage_field<double>, dim3(blocks), dim3(256), 0, c->stream,
+                       tp<double>(c->zb), tp<double>(c->U[c->cur][0]), c->nc, tp<double>(c->stagef));
+  else
+    hipLaunchKernelGGL(k_stage_field<float>, dim3(blocks), dim3(256), 0, c->stream,
+                       tp<float>(c->zb), tp<float>(c->U[c->cur][0]), c->nc, tp<float>(c->stagef));
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MFSV_OK;
 }
@@ -862,6 +984,7 @@ int mfsv_run(mfsv_ctx* c, int64_t max_timesteps, int use_graph) {
   }
   c->stage_ms = 0.0;
   c->stage_launches = 0;
+  c->ran = c->ran || max_timesteps > 0;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   int64_t left = max_timesteps;
   while (left > 0) {

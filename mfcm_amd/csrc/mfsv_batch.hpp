@@ -294,6 +294,8 @@ int mfsv_batch_run(mfsv_batch* b, int64_t max_timesteps, int use_graph) {
   for (auto* c : b->active)
     if (c->nstages != b->nstages || std::memcmp(b->tab, c->tab, sizeof(b->tab)) != 0)
       return bfail(b, MFSV_ERR_STATE, "a member's Runge-Kutta scheme changed after it joined");
+  if (max_timesteps > 0)
+    for (auto* c : b->active) c->ran = true;
   BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
   int64_t left = max_timesteps;
   while (left > 0) {

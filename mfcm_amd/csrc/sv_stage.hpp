@@ -103,6 +103,17 @@ struct StageArgs {
   const T* irate; T* icap;
   const T* sb0; const T* sb1;
   const T* mu;
+  // per-cell Manning parameters (ManningRoughnessSourceTerm.hpp:129-140):
+  // with mfull, nfield is n_shallow and these give n_deep, d_shallow, d_deep
+  int32_t mfull;
+  const T* ndp_f; const T* dsh_f; const T* ddp_f;
+  // output fields of a derivative evaluation (KOUT only; null = not wanted):
+  // face fluxes h, u, v, z ((nx+1)*ny x-faces, then nx*(ny+1) y-faces), the
+  // Manning n and friction slope, the bed slopes.  diag != 0 leaves the
+  // infiltration capacity untouched.
+  int32_t diag;
+  T* dg_fh; T* dg_fu; T* dg_fv; T* dg_fz;
+  T* dg_nh; T* dg_sf; T* dg_zx; T* dg_zy;
 };
 
 template <typename T> struct Row { T h, u, v, z; };
@@ -230,7 +241,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   constexpr uint32_t S = sizeof(T);
   constexpr int NB = NACC > 0 ? NACC : 1;
   Ctrl* __restrict__ C = A.ctrl;
-  if (C->done) return;
+  // a finished device-controlled run skips its remaining captured stages;
+  // a derivative evaluation (KOUT) is always an explicit request
+  if constexpr (!KOUT) {
+    if (C->done) return;
+  }
 
   const int nx = A.nx, ny = A.ny;
   const int strip = tile / A.nrb, rb = tile - strip * A.nrb;
@@ -468,8 +483,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       }
       Xw1 = {F.h, F.n, F.t, F.z};
       if constexpr (KOUT) {
-        if (A.br_x && x >= 0 && x <= nx && lane >= kHalo && lane <= kWave - kHalo)
-          A.br_x[c * (nx + 1) + x] = br;
+        if (x >= 0 && x <= nx && lane >= kHalo && lane <= kWave - kHalo) {
+          const int64_t fi = int64_t(c) * (nx + 1) + x;
+          if (A.br_x) A.br_x[fi] = br;
+          if (A.dg_fh) {
+            A.dg_fh[fi] = Xw1.h; A.dg_fu[fi] = Xw1.u; A.dg_fv[fi] = Xw1.v; A.dg_fz[fi] = Xw1.z;
+          }
+        }
       }
     }
     // ---- south y-face of row c (face id (nx+1)ny + c*nx + x)
@@ -494,7 +514,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       }
       Fs1 = {F.h, F.t, F.n, F.z};
       if constexpr (KOUT) {
-        if (A.br_y && own) A.br_y[c * nx + x] = br;
+        if (own) {
+          if (A.br_y) A.br_y[c * nx + x] = br;
+          if (A.dg_fh) {
+            const int64_t fi = int64_t(nx + 1) * ny + int64_t(c) * nx + x;
+            A.dg_fh[fi] = Fs1.h; A.dg_fu[fi] = Fs1.u; A.dg_fv[fi] = Fs1.v; A.dg_fz[fi] = Fs1.z;
+          }
+        }
       }
     }
     // ---- cell pass + sources + RK for row r = c - 1
@@ -550,11 +576,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       dvdt += dvdt_bed;
       // ManningRoughnessSourceTerm.cpp:55-90
       if (A.manning) {
-        T nsh = A.n_sh, ndp = A.n_dp;
+        T nsh = A.n_sh, ndp = A.n_dp, dsh = A.d_sh, ddp = A.d_dp;
         if (has_n) { nsh = X0.n; ndp = nsh; }
+        if constexpr (EXT) {
+          if (A.mfull) { ndp = ld(A.ndp_f, wo); dsh = ld(A.dsh_f, wo); ddp = ld(A.ddp_f, wo); }
+        }
         T n = nsh;  // mix(a, a, t) == a
         if (nsh != ndp) {
-          T t = vdiv(h - A.d_sh, A.d_dp - A.d_sh);
+          T t = vdiv(h - dsh, ddp - dsh);
           t = fmin_t(fmax_t(t, T(0)), T(1));
           n = nsh + (ndp - nsh) * (t * t * (T(3) - T(2) * t));
         }
@@ -569,6 +598,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         if (fabs_t(pv) > fabs_t(v) && sy_sign(pv) != sy_sign(v)) dvm = div_dt(-v, dtT, rdt);
         dudt = wet ? dudt + dum : dudt;
         dvdt = wet ? dvdt + dvm : dvdt;
+        if constexpr (KOUT) {
+          if (A.dg_nh && own) {
+            st(A.dg_nh, wo, n);
+            st(A.dg_sf, wo, wet ? Sf : T(0));
+          }
+        }
+      }
+      if constexpr (KOUT) {
+        if (A.dg_zx && own) {  // Constants.cpp:27-44: Minmod3 slopes of the bed
+          // the difference form holds dz (exact multiple of the slope for
+          // power-of-two spacing), the quotient form the slope itself
+          st(A.dg_zx, wo, QUICK ? p_zx * rdx : p_zx);
+          st(A.dg_zy, wo, QUICK ? p_zy * rdy : p_zy);
+        }
       }
       if constexpr (EXT) {
         // overshoot cap shared by the velocity sources: a change that would
@@ -593,7 +636,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           const T capv = ld(A.icap, wo);
           di = (di > capv) ? capv : di;
           dhdt -= div_dt(di, dtT, rdt);
-          if (own) st(A.icap, wo, T(capv - di));
+          if (own && !(KOUT && A.diag)) st(A.icap, wo, T(capv - di));
         }
         // EddyViscositySourceTerm.cpp:44-98: Minmod3 of the first-derivative
         // fields (x of du/dx, y of du/dy), Smagorinsky or per-cell mu

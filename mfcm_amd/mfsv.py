@@ -30,7 +30,13 @@ E8_DRY_VELOCITY, E9_VELOCITY_CAP = 0x80, 0x100
 FIELD = {"h": 0, "u": 1, "v": 2, "zb": 3, "manning_n": 4, "qbdy0": 5, "qbdy1": 6,
          "hbdy0": 7, "hbdy1": 8, "dhdt": 9, "dudt": 10, "dvdt": 11,
          "branch_x": 12, "branch_y": 13, "eloss_fx": 14, "eloss_fy": 15,
-         "infil_rate": 16, "infil_cap": 17, "sbdy0": 18, "sbdy1": 19, "eddy_mu": 20}
+         "infil_rate": 16, "infil_cap": 17, "sbdy0": 18, "sbdy1": 19, "eddy_mu": 20,
+         "n_deep": 21, "d_shallow": 22, "d_deep": 23,
+         "flux_h": 24, "flux_u": 25, "flux_v": 26, "flux_z": 27,
+         "mannings_n": 28, "friction_slope": 29, "dzdx_bed": 30, "dzdy_bed": 31,
+         "stage": 32}
+FACE_FIELDS = ("flux_h", "flux_u", "flux_v", "flux_z")
+DIAG_FLUXES, DIAG_MANNING, DIAG_BED_SLOPES = 1, 2, 4
 
 ERRORS = {-1: "bad argument", -2: "HIP runtime error", -3: "invalid state",
           -4: "out of device memory", -5: "unsupported"}
@@ -110,6 +116,8 @@ SIGNATURES = {
     "mfsv_upload": (_I, [_P, _I, _P, _I64]),
     "mfsv_download": (_I, [_P, _I, _P, _I64]),
     "mfsv_dudt": (_I, [_P, _D, _D, _I]),
+    "mfsv_diagnose": (_I, [_P, _I, _D, _D, _I]),
+    "mfsv_update_stage_field": (_I, [_P]),
     "mfsv_timestep": (_I, [_P, _D, _D, C.POINTER(C.c_double)]),
     "mfsv_accept": (_I, [_P]),
     "mfsv_volume": (_I, [_P, C.POINTER(C.c_double)]),
@@ -207,7 +215,18 @@ class Catchment:
         p = Params()
         p.flags = int(sc.flags)
         p.cn_target = float(sc.cn_target)
-        if sc.n_manning is not None:
+        if getattr(sc, "manning_fields", None) is not None:
+            # per-cell n_shallow, n_deep, d_shallow, d_deep
+            p.manning = 1
+            ns, nd, ds, dd = (np.broadcast_to(np.asarray(a, np.float64), (self.ny, self.nx))
+                              for a in sc.manning_fields)
+            self.upload("manning_n", ns)
+            self.upload("n_deep", nd)
+            self.upload("d_shallow", ds)
+            self.upload("d_deep", dd)
+            p.n_shallow, p.n_deep = float(np.mean(ns)), float(np.mean(nd))
+            p.d_shallow, p.d_deep = float(np.mean(ds)), float(np.mean(dd))
+        elif sc.n_manning is not None:
             p.manning = 1
             if isinstance(sc.n_manning, np.ndarray):
                 self.upload("manning_n", sc.n_manning)
@@ -285,6 +304,9 @@ class Catchment:
             n, shp = (self.nx + 1) * self.ny, (self.ny, self.nx + 1)
         elif name == "branch_y":
             n, shp = self.nx * (self.ny + 1), (self.ny + 1, self.nx)
+        elif name in FACE_FIELDS:  # x-faces then y-faces, flat
+            n = (self.nx + 1) * self.ny + self.nx * (self.ny + 1)
+            shp = (n,)
         else:
             n, shp = self.nc, (self.ny, self.nx)
         a = np.empty(n, dtype=np.float64)
@@ -357,6 +379,27 @@ class Catchment:
     def dudt(self, t_sub, dt, branches=False):
         self._check(lib().mfsv_dudt(self._h, t_sub, dt, 1 if branches else 0))
         return self.download("dhdt"), self.download("dudt"), self.download("dvdt")
+
+    def diagnose(self, source=0, t_sub=0.0, dt=1.0, what=7):
+        """Output-only fields (face fluxes, Manning n and friction slope, bed
+        slopes) of U0 (source 0) or of the last RK stage's input (source 1)."""
+        self._check(lib().mfsv_diagnose(self._h, source, t_sub, dt, what))
+        out = {}
+        if what & DIAG_FLUXES:
+            for f in FACE_FIELDS:
+                out[f] = self.download(f)
+        if what & DIAG_MANNING and self.params.manning:
+            out["mannings_n"] = self.download("mannings_n")
+            out["friction_slope"] = self.download("friction_slope")
+        if what & DIAG_BED_SLOPES:
+            out["dzdx_bed"] = self.download("dzdx_bed")
+            out["dzdy_bed"] = self.download("dzdy_bed")
+        return out
+
+    def stage_field(self):
+        """z_bed + h of U0 (the solver's stage field)."""
+        self._check(lib().mfsv_update_stage_field(self._h))
+        return self.download("stage")
 
     def control_timestep(self, t_local, dt):
         cn = C.c_double()

@@ -138,6 +138,7 @@ class Scenario:
     stage_t: np.ndarray | None = None     # stage boundary: water-level series (s, m)
     stage_v: np.ndarray | None = None
     stage_sel: np.ndarray | None = None   # cells of the stage boundary (ny x nx, bool)
+    manning_fields: tuple | None = None   # per-cell (n_shallow, n_deep, d_shallow, d_deep)
     t_end: float = 3600.0
     step_duration: float = 60.0
     initial_dt: float = 1.0

@@ -351,7 +351,9 @@ def sim_from_scenario(sc):
     sim = Sim(sc.nx, sc.ny, sc.dx, sc.dy, sc.scheme, sc.flags, sc.cn_target)
     sim.set_bed(sc.zb)
     sim.set_state(sc.h0, sc.u0, sc.v0)
-    if sc.n_manning is not None:
+    if getattr(sc, "manning_fields", None) is not None:
+        sim.set_manning(*sc.manning_fields)
+    elif sc.n_manning is not None:
         sim.set_manning(sc.n_manning, sc.n_manning, sc.d_shallow, sc.d_deep)
     if sc.rain_t is not None:
         sim.add_discharge(sc.rain_t, sc.rain_v)
