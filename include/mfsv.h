@@ -202,8 +202,10 @@ int mfsv_download(mfsv_ctx* ctx, int field, double* host, int64_t count);
 int mfsv_dudt(mfsv_ctx* ctx, double t_sub, double dt, int want_branches);
 /* Output-only fields of the reference's solver (the fields that
  * MultiFieldOutputFile and the check files write, Solver.hpp:124-161):
- * one derivative evaluation without side effects (the infiltration capacity
- * is not consumed) that fills the selected output fields.
+ * one derivative evaluation without side effects that fills the selected
+ * output fields.  They depend on the state, the bed and the Manning
+ * parameters only, so boundary and further source-term fields need not be
+ * set (the infiltration capacity is not consumed).
  * source 0: the current state U0 at stage time t_sub with timestep dt.
  * source 1: the input of the last RK stage of the last timestep -- the state
  *   the reference's last update_dUdt saw, whose fluxes, Manning n and
@@ -237,6 +239,12 @@ int mfsv_run_begin(mfsv_ctx* ctx);
  * kernels with HIP events (read with mfsv_last_stage_ms). */
 int mfsv_run(mfsv_ctx* ctx, int64_t max_timesteps, int graph);
 int mfsv_get_stats(mfsv_ctx* ctx, mfsv_stats* out);
+/* The inner loop of one outer step (TemporalScheme::step,
+ * TemporalScheme.hpp:112-149) on the device: after mfsv_run_begin and the
+ * step's mfsv_start_new_step, attempt timesteps until the current outer step
+ * is complete, so the host can prepare the next step's boundary fields
+ * between steps.  No-op when every outer step is complete. */
+int mfsv_run_step(mfsv_ctx* ctx, int graph);
 
 /* Timing of the last mfsv_run: milliseconds measured with HIP events on the
  * context's stream around the launches of that call. */

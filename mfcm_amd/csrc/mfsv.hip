@@ -332,6 +332,10 @@ void launch_diag_typed(mfsv_ctx* c, int s, bool prev_u0, int what) {
     A.in_h = tp<T>(c->U[o][0]); A.in_u = tp<T>(c->U[o][1]); A.in_v = tp<T>(c->U[o][2]);
   }
   A.diag = 1;
+  // the output fields depend on the state, the bed and the Manning
+  // parameters only: no boundary or further source field is read
+  A.discharge = 0; A.head = 0; A.stageb = 0;
+  A.eloss = 0; A.infil = 0; A.eddy = 0;
   if (what & MFSV_DIAG_FLUXES) {
     A.dg_fh = tp<T>(c->dgf[0]); A.dg_fu = tp<T>(c->dgf[1]);
     A.dg_fv = tp<T>(c->dgf[2]); A.dg_fz = tp<T>(c->dgf[3]);
@@ -821,8 +825,11 @@ int mfsv_diagnose(mfsv_ctx* c, int source, double t_sub, double dt, int what) {
   if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
   if (source == 1 && !c->ran)
     return fail(c, MFSV_ERR_STATE, "no timestep has run: there is no last stage input");
+  if (!c->zb.p || !c->U[0][0].p) return fail(c, MFSV_ERR_STATE, "bed and state must be uploaded");
+  const int mset = (c->ndpf.p ? 1 : 0) + (c->dshf.p ? 1 : 0) + (c->ddpf.p ? 1 : 0);
+  if (mset != 0 && (mset != 3 || !c->nfield_set))
+    return fail(c, MFSV_ERR_STATE, "per-cell Manning needs MANNING_N, N_DEEP, D_SHALLOW and D_DEEP");
   int rc;
-  if ((rc = check_inputs(c))) return rc;
   if ((rc = read_ctrl(c))) return rc;
   c->cur = c->hctrl->cur;
   const double keep_t = c->hctrl->t_local, keep_dt = c->hctrl->dt;
@@ -1019,6 +1026,30 @@ int mfsv_run(mfsv_ctx* c, int64_t max_timesteps, int use_graph) {
   c->last_ms = ms;
   if ((rc = read_ctrl(c))) return rc;
   c->cur = c->hctrl->cur;
+  return MFSV_OK;
+}
+
+int mfsv_run_step(mfsv_ctx* c, int use_graph) {
+  if (!c || use_graph < 0 || use_graph > 2) return MFSV_ERR_ARG;
+  if (c->batch) return fail(c, MFSV_ERR_STATE, "the context is a batch member: remove it first");
+  if (!c->begun) return fail(c, MFSV_ERR_STATE, "mfsv_run_begin must come first");
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  int rc;
+  if ((rc = read_ctrl(c))) return rc;
+  const int64_t total = c->hctrl->nsteps;
+  if (c->hctrl->step >= total) return MFSV_OK;
+  // the device controller finishes the run at the end of this outer step
+  c->hctrl->nsteps = c->hctrl->step + 1;
+  c->hctrl->done = 0;
+  HIPCHK(c, hipMemcpyAsync(c->ctrl.p, c->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+  rc = mfsv_run(c, int64_t(1) << 50, use_graph);
+  const int rc2 = read_ctrl(c);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  c->hctrl->nsteps = total;
+  c->hctrl->done = c->hctrl->step >= total ? 1 : 0;
+  HIPCHK(c, hipMemcpyAsync(c->ctrl.p, c->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MFSV_OK;
 }
 

@@ -192,6 +192,14 @@ class RungeKuttaSolver : public RungeKuttaTemporalScheme<typename Solver::TimeTy
 
   std::shared_ptr<Solver> solver() const { return solver_; }
 
+  // The inner loop on the device when the solver runs it there (the same
+  // controller and landing rules, evaluated by the device).
+  typename TypedTemporalScheme<TimeType>::step_result step() override {
+    if (!solver_->device_loop()) return TypedTemporalScheme<TimeType>::step();
+    const auto r = solver_->run_device_step();
+    return {r.first, r.second};
+  }
+
   double get_latest_control_number(const TimeType& dt) override {
     return solver_->control_number(this->coeffs().nsteps(), dt);
   }

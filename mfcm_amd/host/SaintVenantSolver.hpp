@@ -20,6 +20,7 @@
 #pragma once
 
 #include <cmath>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -43,6 +44,8 @@ class TimeSeries {
       throw std::runtime_error("TimeSeries: need matching times and values, at least 2 samples");
   }
   bool empty() const { return t_.empty(); }
+  const std::vector<double>& times() const { return t_; }
+  const std::vector<double>& values() const { return v_; }
   double at(double time) const {
     size_t i1 = 1;
     while (i1 < t_.size() && !(t_[i1] > time)) ++i1;
@@ -54,6 +57,21 @@ class TimeSeries {
 
  private:
   std::vector<double> t_, v_;
+};
+
+// One entry of a boundary section (DischargeBoundarySourceTerm.hpp:114-148,
+// HeadBoundarySourceTerm.hpp, StageBoundarySourceTerm.hpp): a constant or a
+// time series on a set of cells (ids ascending; `global` = every cell).  Per
+// outer step a discharge entry adds its endpoints to the cells' endpoint
+// fields, a head or stage entry sets them.
+struct BoundaryEntry {
+  enum Kind { Discharge, Head, Stage } kind = Discharge;
+  std::string name;
+  bool constant = true;
+  double value = 0.0;
+  TimeSeries series;
+  bool global = true;
+  std::vector<int64_t> cells;
 };
 
 // What the reference reads from its config sections (source terms,
@@ -83,6 +101,15 @@ struct SaintVenantOptions {
   // stage_cells is non-zero (empty series = off)
   TimeSeries stage;
   std::vector<uint8_t> stage_cells;
+  // per-cell n_deep, d_shallow, d_deep (ManningRoughnessSourceTerm.hpp:
+  // 129-140); with them manning_n is n_shallow
+  std::vector<double> manning_n_deep, manning_d_shallow, manning_d_deep;
+  // boundary sections, in the order they apply
+  std::vector<BoundaryEntry> boundaries;
+  // inner loop of each outer step on the device (mfsv_run_step) instead of
+  // one host round trip per timestep; graph: 1 = hipGraph replay
+  bool device_loop = false;
+  int graph = 1;
 };
 
 template <typename TT, typename T>
@@ -121,16 +148,25 @@ class SaintVenantSolver {
     p.manning = opt.manning ? 1 : 0;
     p.n_shallow = opt.n_shallow; p.n_deep = opt.n_deep;
     p.d_shallow = opt.d_shallow; p.d_deep = opt.d_deep;
-    p.discharge = opt.rain.empty() ? 0 : 1;
-    p.head = opt.head_cols > 0 ? 1 : 0;
-    p.head_x0 = 0; p.head_x1 = opt.head_cols;
+    bool q_any = !opt.rain.empty(), q_cells = false, h_any = false, h_cells = false, s_any = false;
+    for (const auto& b : opt.boundaries) {
+      if (b.kind == BoundaryEntry::Discharge) { q_any = true; q_cells |= !b.global; }
+      if (b.kind == BoundaryEntry::Head) { h_any = true; h_cells |= !b.global; }
+      if (b.kind == BoundaryEntry::Stage) s_any = true;
+    }
+    if (h_any && opt.head_cols > 0)
+      throw std::runtime_error("SaintVenantOptions: head_cols and head boundary entries together");
+    p.discharge = q_any ? (q_cells ? 2 : 1) : 0;
+    p.head = opt.head_cols > 0 ? 1 : (h_any ? (h_cells ? 2 : 1) : 0);
+    p.head_x0 = 0; p.head_x1 = opt.head_cols > 0 ? opt.head_cols : (h_any ? mesh.nx() : 0);
     p.h_dry = opt.h_dry; p.u_max = opt.u_max;
     p.energy_loss = opt.eloss_fx.empty() ? 0 : 1;
     p.infiltration = opt.infil_rate.empty() ? 0 : 1;
-    p.stage = opt.stage.empty() ? 0 : 1;
+    p.stage = (opt.stage.empty() && !s_any) ? 0 : 1;
     p.eddy = opt.eddy;
     p.mu_const = opt.mu_const; p.mu_smag = opt.mu_smag;
     check(mfsv_set_params(c, &p), c, "mfsv_set_params");
+    params_ = p;
     check(mfsv_set_time(c, tparams->start_time(), tparams->end_time(), tparams->step_duration(),
                         tparams->initial_timestep()), c, "mfsv_set_time");
     if (opt.tile_rows > 0) check(mfsv_set_tile_rows(c, opt.tile_rows), c, "mfsv_set_tile_rows");
@@ -140,6 +176,10 @@ class SaintVenantSolver {
       DataArray<double> nf(n);
       bind_and_upload(nf, MFSV_FIELD_MANNING_N, opt.manning_n);
     }
+    const int mfields = int(!opt.manning_n_deep.empty()) + int(!opt.manning_d_shallow.empty()) +
+                        int(!opt.manning_d_deep.empty());
+    if (mfields != 0 && (mfields != 3 || opt.manning_n.empty()))
+      throw std::runtime_error("per-cell Manning needs manning_n and all of n_deep, d_shallow, d_deep");
     auto extra = [&](const std::vector<double>& f, int id) {
       if (f.empty()) return;
       if (f.size() != n) throw std::runtime_error("source-term field size != cell count");
@@ -151,6 +191,9 @@ class SaintVenantSolver {
     extra(opt.infil_rate, MFSV_FIELD_INFIL_RATE);
     extra(opt.infil_capacity, MFSV_FIELD_INFIL_CAP);
     extra(opt.eddy_mu, MFSV_FIELD_EDDY_MU);
+    extra(opt.manning_n_deep, MFSV_FIELD_N_DEEP);
+    extra(opt.manning_d_shallow, MFSV_FIELD_D_SHALLOW);
+    extra(opt.manning_d_deep, MFSV_FIELD_D_DEEP);
     bind_and_upload(U_.h.data(), MFSV_FIELD_H, h0);
     bind_and_upload(U_.u.data(), MFSV_FIELD_U, u0);
     bind_and_upload(U_.v.data(), MFSV_FIELD_V, v0);
@@ -190,31 +233,104 @@ class SaintVenantSolver {
 
   // Solver.cpp:119-129: boundary endpoints for [time_now, time_now + step).
   void start_new_step(TT time_now) {
-    double q0 = 0.0, q1 = 0.0;
-    if (!opt_.rain.empty()) {
-      q0 = 0.0 + opt_.rain.at(double(time_now));
-      q1 = 0.0 + opt_.rain.at(double(time_now + tparams_->step_duration()));
-    }
-    if (!opt_.stage.empty()) {  // StageBoundarySourceTerm.hpp:114-142
-      const double nan = std::numeric_limits<double>::quiet_NaN();
-      const double a0 = opt_.stage.at(double(time_now));
-      const double a1 = opt_.stage.at(double(time_now + tparams_->step_duration()));
-      const size_t nc = mesh_.cell_count();
-      std::vector<double> s0(nc), s1(nc);
-      for (size_t i = 0; i < nc; ++i) {
-        const bool on = opt_.stage_cells.empty() || opt_.stage_cells[i];
-        s0[i] = on ? a0 : nan;
-        s1[i] = on ? a1 : nan;
-      }
-      check(mfsv_upload(ctx_->get(), MFSV_FIELD_SBDY0, s0.data(), int64_t(nc)), ctx_->get(), "mfsv_upload");
-      check(mfsv_upload(ctx_->get(), MFSV_FIELD_SBDY1, s1.data(), int64_t(nc)), ctx_->get(), "mfsv_upload");
-    }
-    const double hb = opt_.head_cols > 0 ? 0.0 : std::numeric_limits<double>::quiet_NaN();
-    check(mfsv_start_new_step(ctx_->get(), double(time_now), q0, q1, hb, hb), ctx_->get(),
+    const Endpoints e = endpoints(double(time_now));
+    const size_t nc = mesh_.cell_count();
+    mfsv_ctx* c = ctx_->get();
+    auto up = [&](int field, const std::vector<double>& v, std::vector<double>& last) {
+      if (v.empty() || v == last) return;
+      check(mfsv_upload(c, field, v.data(), int64_t(nc)), c, "mfsv_upload");
+      last = v;
+    };
+    up(MFSV_FIELD_QBDY0, e.qc0, up_[0]);
+    up(MFSV_FIELD_QBDY1, e.qc1, up_[1]);
+    up(MFSV_FIELD_HBDY0, e.hc0, up_[2]);
+    up(MFSV_FIELD_HBDY1, e.hc1, up_[3]);
+    up(MFSV_FIELD_SBDY0, e.sc0, up_[4]);
+    up(MFSV_FIELD_SBDY1, e.sc1, up_[5]);
+    last_ = e;
+    // the device loop starts once the step's per-cell fields are in place
+    if (opt_.device_loop && !begun_) begin_device_run();
+    check(mfsv_start_new_step(c, double(time_now), e.q0, e.q1, e.hb0, e.hb1), c,
           "mfsv_start_new_step");
   }
-  void end_of_step(TT /*time_now*/) {}
-  void do_outputs(TT /*time_now*/) {}
+  // Endpoint values of one outer step.  Scalars: global discharge and head;
+  // fields: per-cell discharge (summed), head and stage (set; NaN = none).
+  struct Endpoints {
+    double q0 = 0.0, q1 = 0.0;
+    double hb0 = std::numeric_limits<double>::quiet_NaN(), hb1 = std::numeric_limits<double>::quiet_NaN();
+    std::vector<double> qc0, qc1, hc0, hc1, sc0, sc1;
+  };
+  Endpoints endpoints(double time_now) const {
+    Endpoints e;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const double t1 = time_now + double(tparams_->step_duration());
+    const size_t nc = mesh_.cell_count();
+    if (!opt_.rain.empty()) {
+      e.q0 = 0.0 + opt_.rain.at(time_now);
+      e.q1 = 0.0 + opt_.rain.at(t1);
+    }
+    if (opt_.head_cols > 0) e.hb0 = e.hb1 = 0.0;
+    const mfsv_params& p = params_;
+    if (p.discharge == 2) { e.qc0.assign(nc, 0.0); e.qc1.assign(nc, 0.0); }
+    if (p.head == 2) { e.hc0.assign(nc, nan); e.hc1.assign(nc, nan); }
+    if (p.stage && opt_.stage.empty()) { e.sc0.assign(nc, nan); e.sc1.assign(nc, nan); }
+    for (const auto& b : opt_.boundaries) {
+      const double v0 = b.constant ? b.value : b.series.at(time_now);
+      const double v1 = b.constant ? b.value : b.series.at(t1);
+      auto each = [&](auto f) {
+        if (b.global) for (size_t i = 0; i < nc; ++i) f(i);
+        else for (int64_t i : b.cells) f(size_t(i));
+      };
+      switch (b.kind) {
+        case BoundaryEntry::Discharge:
+          if (p.discharge == 2) each([&](size_t i) { e.qc0[i] += v0; e.qc1[i] += v1; });
+          else { e.q0 += v0; e.q1 += v1; }
+          break;
+        case BoundaryEntry::Head:
+          if (p.head == 2) each([&](size_t i) { e.hc0[i] = v0; e.hc1[i] = v1; });
+          else { e.hb0 = v0; e.hb1 = v1; }
+          break;
+        case BoundaryEntry::Stage:
+          each([&](size_t i) { e.sc0[i] = v0; e.sc1[i] = v1; });
+          break;
+      }
+    }
+    if (!opt_.stage.empty()) {  // StageBoundarySourceTerm.hpp:114-142
+      const double a0 = opt_.stage.at(time_now), a1 = opt_.stage.at(t1);
+      e.sc0.assign(nc, nan);
+      e.sc1.assign(nc, nan);
+      for (size_t i = 0; i < nc; ++i) {
+        const bool on = opt_.stage_cells.empty() || opt_.stage_cells[i];
+        e.sc0[i] = on ? a0 : nan;
+        e.sc1[i] = on ? a1 : nan;
+      }
+    }
+    return e;
+  }
+  const Endpoints& last_endpoints() const { return last_; }
+
+  // The inner loop of the current outer step on the device; returns the
+  // accepted and repeated timesteps it took.
+  std::pair<size_t, size_t> run_device_step() {
+    mfsv_ctx* c = ctx_->get();
+    mfsv_stats a{}, b{};
+    check(mfsv_get_stats(c, &a), c, "mfsv_get_stats");
+    check(mfsv_run_step(c, opt_.graph), c, "mfsv_run_step");
+    check(mfsv_get_stats(c, &b), c, "mfsv_get_stats");
+    return {size_t(b.accepted - a.accepted), size_t(b.rejected - a.rejected)};
+  }
+  bool device_loop() const { return opt_.device_loop; }
+  void set_end_of_step_hook(std::function<void(double)> f) { eos_hook_ = std::move(f); }
+  void set_output_hook(std::function<void(double)> f) { out_hook_ = std::move(f); }
+
+  // Solver.hpp:90-98: the stage field z_bed + h, then the measures.
+  void end_of_step(TT time_now) {
+    check(mfsv_update_stage_field(ctx_->get()), ctx_->get(), "mfsv_update_stage_field");
+    if (eos_hook_) eos_hook_(double(time_now));
+  }
+  void do_outputs(TT time_now) {
+    if (out_hook_) out_hook_(double(time_now));
+  }
 
   // The current state U0, downloaded into the host mirrors.
   State& state() {
@@ -246,8 +362,30 @@ class SaintVenantSolver {
     a.move_to_device();
   }
 
+  // The device-controlled loop needs the scalar endpoints of every outer step
+  // up front (a forcing table) and the controller initialised once.
+  void begin_device_run() {
+    mfsv_ctx* c = ctx_->get();
+    const size_t ns = tparams_->num_steps();
+    std::vector<double> table(4 * ns);
+    for (size_t i = 0; i < ns; ++i) {
+      const double t = double(tparams_->start_time()) + double(i) * double(tparams_->step_duration());
+      const Endpoints e = endpoints(t);
+      table[4 * i] = e.q0; table[4 * i + 1] = e.q1;
+      table[4 * i + 2] = e.hb0; table[4 * i + 3] = e.hb1;
+    }
+    check(mfsv_set_forcing_table(c, int64_t(ns), table.data()), c, "mfsv_set_forcing_table");
+    check(mfsv_run_begin(c), c, "mfsv_run_begin");
+    begun_ = true;
+  }
+
   Cartesian2DMesh mesh_;
   SaintVenantOptions opt_;
+  mfsv_params params_{};
+  bool begun_ = false;
+  std::vector<double> up_[6];
+  Endpoints last_;
+  std::function<void(double)> eos_hook_, out_hook_;
   std::shared_ptr<TimeParameters<TT>> tparams_;
   size_t nstages_;
   std::shared_ptr<Context> ctx_;

@@ -65,8 +65,8 @@ class TypedTemporalScheme : public TemporalScheme {
   size_t total_repeated_timesteps() const { return total_.num_repeated_timesteps; }
   void set_verbose(bool v) { verbose_ = v; }
 
-  // TemporalScheme.hpp:112-149
-  step_result step() {
+  // TemporalScheme.hpp:112-149 (a solver may run this loop on the device)
+  virtual step_result step() {
     step_result result{0, 0};
     TimeType t_local = 0.0;
     const TimeType step_duration = time_params_->step_duration();

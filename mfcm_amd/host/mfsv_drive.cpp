@@ -1,6 +1,25 @@
 // mfsv_drive.cpp -- command-line driver of the C++ mirror, in the shape of
 // the reference's main (mfcm.cpp:19-49): build a RungeKuttaSolver over a
-// SaintVenantSolver, call solve(), and write the final state.
+// SaintVenantSolver, call solve(), and write the outputs.
+//
+//   mfsv_drive [options] <config-file>
+//
+// reads a configuration in the reference's mf format (GlobalConfig::init,
+// Config.cpp:102-174: the simulation base path is --base-path or the
+// file's directory) and runs it as mfcm would, writing the configured
+// output files, check files and measures under the base path.  Options:
+//   --base-path P        simulation base path
+//   --accel-device N     device index (also "device" in the device section)
+//   --precision f64|f32  value precision (default f64, the plain method)
+//   --host-loop          one host round trip per timestep (the reference's
+//                        loop shape) instead of each outer step's inner loop
+//                        on the device
+//   --eager              launch kernels one by one instead of graph replay
+//   --flags N            MFSV_E* extension bits (default 0)
+//   --quiet              no progress lines
+//   --dump-state DIR     also write the final h, u, v (raw float64) and
+//                        stats.txt to DIR, as the array mode below does
+// and prints "solved: N timesteps, M repeated" at the end.
 //
 //   mfsv_drive <input-dir> <output-dir>
 //
@@ -27,6 +46,7 @@
 #include <string>
 #include <vector>
 
+#include "ConfiguredModel.hpp"
 #include "RungeKutta.hpp"
 #include "SaintVenantSolver.hpp"
 
@@ -135,12 +155,91 @@ int run(const std::map<std::string, std::string>& kv, const std::string& in, con
   return 0;
 }
 
+template <typename T>
+int run_config(GlobalConfig& gc, const RunOptions& ro, const std::string& dump) {
+  ConfiguredModel<T> model(gc, ro);
+  model.solve();
+  auto& s = model.scheme();
+  if (!dump.empty()) {
+    auto& U = model.solver().state();
+    const size_t n = model.mesh().cell_count();
+    write_f64(dump + "/h.f64", U.h.data().data(), n);
+    write_f64(dump + "/u.f64", U.u.data().data(), n);
+    write_f64(dump + "/v.f64", U.v.data().data(), n);
+    std::ofstream st(dump + "/stats.txt");
+    st.precision(17);
+    st << "timesteps = " << s.total_timesteps() << "\n"
+       << "repeated_timesteps = " << s.total_repeated_timesteps() << "\n"
+       << "volume = " << model.solver().volume() << "\n"
+       << "precision = " << (ro.precision == MFSV_F64 ? "f64" : "f32") << "\n";
+    if (!st) throw std::runtime_error("cannot write stats.txt");
+  }
+  std::cout << "solved: " << s.total_timesteps() << " timesteps, " << s.total_repeated_timesteps()
+            << " repeated" << std::endl;
+  return 0;
+}
+
+void usage(const char* prog) {
+  std::cerr << "usage: " << prog << " [--base-path P] [--accel-device N] [--precision f64|f32]"
+            << " [--host-loop] [--eager] [--flags N] [--quiet] [--dump-state DIR] <config-file>\n"
+            << "       " << prog << " <input-dir> <output-dir>" << std::endl;
+}
+
+int main_config(int argc, char* argv[]) {
+  std::string base, file, prec = "f64", dump;
+  RunOptions ro;
+  bool device_given = false;
+  for (int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    auto next = [&]() -> std::string {
+      if (i + 1 >= argc) throw std::runtime_error("option " + a + " needs a value");
+      return argv[++i];
+    };
+    if (a == "--base-path") base = next();
+    else if (a == "--accel-device") { ro.device = std::stoi(next()); device_given = true; }
+    else if (a == "--accel-platform") (void)next();
+    else if (a == "--precision") prec = next();
+    else if (a == "--host-loop") ro.device_loop = false;
+    else if (a == "--eager") ro.graph = 0;
+    else if (a == "--flags") ro.flags = int32_t(std::stol(next(), nullptr, 0));
+    else if (a == "--quiet") ro.quiet = true;
+    else if (a == "--dump-state") dump = next();
+    else if (a == "--help") { usage(argv[0]); return 2; }
+    else if (!a.empty() && a[0] == '-') throw std::runtime_error("unknown option " + a);
+    else if (file.empty()) file = a;
+    else throw std::runtime_error("Multiple input files not yet supported.");
+  }
+  if (file.empty()) throw std::runtime_error("No input configuration specified.");
+  GlobalConfig gc = GlobalConfig::from_file(file, base);
+  if (!device_given) {
+    if (const Config* d = gc.root().find("device")) ro.device = d->get<int>("device", 0);
+  }
+  if (prec == "f64") {
+    ro.precision = MFSV_F64;
+    return run_config<double>(gc, ro, dump);
+  }
+  if (prec == "f32") {
+    ro.precision = MFSV_F32;
+    return run_config<float>(gc, ro, dump);
+  }
+  throw std::runtime_error("precision must be f64 or f32");
+}
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
-  if (argc != 3) {
-    std::cerr << "usage: " << argv[0] << " <input-dir> <output-dir>" << std::endl;
+  if (argc < 2) {
+    usage(argv[0]);
     return 2;
+  }
+  // two plain arguments: the array mode; otherwise a configuration file
+  if (!(argc == 3 && argv[1][0] != '-' && argv[2][0] != '-')) {
+    try {
+      return main_config(argc, argv);
+    } catch (const std::exception& e) {
+      std::cerr << "mfsv_drive: " << e.what() << std::endl;
+      return 1;
+    }
   }
   try {
     const std::string in = argv[1], out = argv[2];

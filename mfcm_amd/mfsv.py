@@ -124,6 +124,7 @@ SIGNATURES = {
     "mfsv_run_begin": (_I, [_P]),
     "mfsv_run": (_I, [_P, _I64, _I]),
     "mfsv_get_stats": (_I, [_P, C.POINTER(Stats)]),
+    "mfsv_run_step": (_I, [_P, _I]),
     "mfsv_last_run_ms": (_I, [_P, C.POINTER(C.c_double)]),
     "mfsv_last_stage_ms": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mfsv_batch_create": (_I, [_I, _I, C.POINTER(_P)]),
@@ -459,6 +460,12 @@ class Catchment:
         per-timestep HIP events around the stage kernels."""
         mode = int(graph) if not isinstance(graph, bool) else (1 if graph else 0)
         self._check(lib().mfsv_run(self._h, int(max_timesteps), mode))
+        return self.stats()
+
+    def run_step(self, graph=True):
+        """The current outer step's inner loop on the device."""
+        mode = int(graph) if not isinstance(graph, bool) else (1 if graph else 0)
+        self._check(lib().mfsv_run_step(self._h, mode))
         return self.stats()
 
     def last_stage_ms(self):

@@ -145,3 +145,44 @@ def test_driver_equals_python_front_end_fp32(gpu_lib, tmp_path):
         np.testing.assert_array_equal(a, b)
     assert (st["timesteps"], st["repeated_timesteps"]) == (cat.n_acc, cat.n_rej)
     cat.close()
+
+
+_CFG = """mesh == cartesian
+{
+  cell count == 8, 6
+  cell size == 2.0, 2.0
+}
+scheme == runge-kutta
+{
+  time unit == s
+  end time == 60
+}
+"""
+
+
+@pytest.mark.parametrize("extra,msg", [
+    ("source term == magic\n", "Unknown source term type"),
+    ("source term == manning roughness\nsource term == manning roughness\n", "configured twice"),
+    ("boundary == head\nboundary == discharge\n", "order discharge, head, stage"),
+    ("boundary == discharge\nboundary == discharge\n", "more than one"),
+    ("boundary == flood\n", "Unknown boundary type"),
+    ("boundary == discharge\n{\n  time series == r\n  {\n    values == nope\n    cells == global\n"
+     "  }\n}\n", "Could not find time series 'nope'"),
+    ("output == o\n{\n  at == edges\n  every seconds == 1\n}\n", "output.at"),
+])
+def test_driver_config_mode_reports_configuration_errors(tmp_path, extra, msg):
+    """Configuration mode: errors in the configuration end the run with a
+    message and status 1 before any device work."""
+    exe = _driver()
+    cfg = tmp_path / "a.mf"
+    cfg.write_text(_CFG + extra)
+    r = subprocess.run([exe, "--quiet", str(cfg)], capture_output=True, text=True)
+    assert r.returncode == 1 and msg in r.stderr, (r.stdout, r.stderr)
+
+
+def test_driver_config_mode_usage(tmp_path):
+    exe = _driver()
+    r = subprocess.run([exe, "--bogus", "x.mf"], capture_output=True, text=True)
+    assert r.returncode == 1 and "unknown option" in r.stderr
+    r = subprocess.run([exe, "--quiet", str(tmp_path / "none.mf")], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot open file" in r.stderr
