@@ -443,6 +443,7 @@ def cpu_baseline(sc, budget_s):
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     sim = O.sim_from_scenario(sc)
+    step = 0
     sim.start_new_step(0.0)
     dt, t_local = sc.initial_dt, 0.0
     n_acc = n_att = 0
@@ -455,13 +456,17 @@ def cpu_baseline(sc, budget_s):
             t_local += dt
             n_acc += 1
         dt = new_dt
-        if t_local + dt > sc.step_duration:
+        el = time.perf_counter() - t0
+        if (el > budget_s and n_acc >= 2) or n_att >= 100000:
+            break
+        if t_local >= sc.step_duration:  # next outer step (TemporalScheme.hpp:244-276)
+            step += 1
+            t_local = 0.0
+            sim.start_new_step(step * sc.step_duration)
+        elif t_local + dt > sc.step_duration:
             dt = sc.step_duration - t_local
         elif t_local + 2 * dt > sc.step_duration:
             dt = (sc.step_duration - t_local) * 0.4
-        el = time.perf_counter() - t0
-        if (el > budget_s and n_acc >= 2) or t_local >= sc.step_duration or n_att >= 10000:
-            break
     return {
         "value": n_acc * sc.ncells / el,
         "unit": "cell-updates/s",
