@@ -277,12 +277,21 @@ __device__ __forceinline__ float fsqrt_t(float a) { return __builtin_amdgcn_sqrt
 // sign * min(2|ql|, 2|qr|, |qc|/2) when ql and qr share a sign (then qc
 // does too), else 0.  One-sided at an edge: the reference reduces to the
 // remaining quotient.
+//
+// The two-argument minmod is the median of a, b and 0 (the operand nearer
+// zero when they share a sign, else zero), so Minmod3 is
+// med3(2 med3(ql, qr, 0), qc/2, 0): the same operand is selected and the
+// scalings are exact; only the sign of a zero result may differ, which no
+// later operation distinguishes (a zero slope enters sums and products only).
+__device__ __forceinline__ float minmod2(float a, float b) {
+  return __builtin_amdgcn_fmed3f(a, b, 0.0f);
+}
+__device__ __forceinline__ double minmod2(double a, double b) {
+  return __builtin_fmax(__builtin_fmin(a, b), __builtin_fmin(__builtin_fmax(a, b), 0.0));
+}
 template <typename T>
 __device__ __forceinline__ T minmod_q(T ql, T qr, T qc, bool hasl, bool hasr) {
-  const bool same = (ql > T(0) && qr > T(0)) || (ql < T(0) && qr < T(0));
-  // min(2|ql|, 2|qr|, |qc|/2) == 2 min(|ql|, |qr|, |qc|/4): power-of-two scalings
-  const T m = T(2) * fmin_t(fmin_t(fabs_t(ql), fabs_t(qr)), T(0.25) * fabs_t(qc));
-  const T in = same ? copysign_t(m, ql) : T(0);
+  const T in = minmod2(T(2) * minmod2(ql, qr), T(0.5) * qc);
   return hasl ? (hasr ? in : ql) : (hasr ? qr : T(0));
 }
 
@@ -301,6 +310,14 @@ template <bool QUICK, typename T>
 __device__ __forceinline__ T dv(T a, T b, T rb) {
   if constexpr (QUICK) return a * rb;
   else return div_c(a, b, rb);
+}
+
+// a / d + b / e: in the difference form a * rd + b * re with the first
+// product fused in FP32 (FP64 rounds both products, as the reference does)
+template <bool QUICK, typename T>
+__device__ __forceinline__ T dv2(T a, T d, T rd, T b, T e, T re) {
+  if constexpr (QUICK) return mad_t(a, rd, b * re);
+  else return div_c(a, d, rd) + div_c(b, e, re);
 }
 
 // Ghost side of a closed edge face (FluxKernel.cpp:109-157 with the other
@@ -351,13 +368,15 @@ __device__ __forceinline__ Flux<T> face_fast(const Side<T>& L, const Side<T>& R,
     const T cp = wp ? fsqrt_t(ghp) : T(0);
     const T gm = wm ? ghm : T(0);
     const T gp = wp ? (wm ? -ghp : ghp) : T(0);
-    const T nfm = unme * (T(0.5) * unme) + gm;
-    const T nfp = unpe * (T(0.5) * unpe) + gp;
+    // mad_t: the FP64 form rounds the product (the reference's operation
+    // order), the FP32 form fuses it
+    const T nfm = mad_t(unme, T(0.5) * unme, gm);
+    const T nfp = mad_t(unpe, T(0.5) * unpe, gp);
     const T a = fmax_t(rus_speed(unpe, cp), rus_speed(unme, cm));
     const T ha = T(0.5) * a;
-    F.h = T(0.5) * (hpe * unpe + hme * unme) - ha * (hpe - hme);
-    F.n = T(0.5) * (nfp + nfm) - ha * (unpe - unme);
-    F.t = T(0.5) * (utpe * unpe + utme * unme) - ha * (utpe - utme);
+    F.h = mad_t(-ha, hpe - hme, T(0.5) * mad_t(hpe, unpe, hme * unme));
+    F.n = mad_t(-ha, unpe - unme, T(0.5) * (nfp + nfm));
+    F.t = mad_t(-ha, utpe - utme, T(0.5) * mad_t(utpe, unpe, utme * unme));
     F.z = (wm && wp) ? (zm - zp) * G : dv<QUICK>(wm ? hp : hm, d, rd);
     *branch = (wm && wp) ? 1.0f : (wm ? 3.0f : 4.0f);
   } else {

@@ -548,11 +548,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       const uint32_t wo = lane_off + uint32_t(r) * row_bytes;
       const T h = R0.h, u = R0.u, v = R0.v;
       // TemporalDerivativeKernel.cpp:57-90
-      T dhdt = dv<QUICK>(Xw0.h - Xe0.h, dx, rdx) + dv<QUICK>(Fs0.h - Fs1.h, dy, rdy);
-      T dudt = dv<QUICK>(Xw0.u - Xe0.u, dx, rdx) + dv<QUICK>(Fs0.u - Fs1.u, dy, rdy);
-      T dvdt = dv<QUICK>(Xw0.v - Xe0.v, dx, rdx) + dv<QUICK>(Fs0.v - Fs1.v, dy, rdy);
-      dudt += p_hx * gx;
-      dvdt += p_hy * gy;
+      T dhdt = dv2<QUICK>(Xw0.h - Xe0.h, dx, rdx, Fs0.h - Fs1.h, dy, rdy);
+      T dudt = dv2<QUICK>(Xw0.u - Xe0.u, dx, rdx, Fs0.u - Fs1.u, dy, rdy);
+      T dvdt = dv2<QUICK>(Xw0.v - Xe0.v, dx, rdx, Fs0.v - Fs1.v, dy, rdy);
+      dudt = mad_t(p_hx, gx, dudt);
+      dvdt = mad_t(p_hy, gy, dvdt);
       // bed slope limited to |dz/dx| <= h/dx
       T dudt_bed, dvdt_bed;
       if constexpr (QUICK) {
@@ -570,8 +570,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         dudt_bed = T(-kG) * dzdx;
         dvdt_bed = T(-kG) * dzdy;
       }
-      dudt += dv<QUICK>(Xw0.z - Xe0.z, dx, rdx);
-      dvdt += dv<QUICK>(Fs0.z - Fs1.z, dy, rdy);
+      if constexpr (QUICK) {
+        dudt = mad_t(Xw0.z - Xe0.z, rdx, dudt);
+        dvdt = mad_t(Fs0.z - Fs1.z, rdy, dvdt);
+      } else {
+        dudt += div_c(Xw0.z - Xe0.z, dx, rdx);
+        dvdt += div_c(Fs0.z - Fs1.z, dy, rdy);
+      }
       dudt += dudt_bed;
       dvdt += dvdt_bed;
       // ManningRoughnessSourceTerm.cpp:55-90
@@ -585,11 +590,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         if (nsh != ndp) {
           T t = vdiv(h - dsh, ddp - dsh);
           t = fmin_t(fmax_t(t, T(0)), T(1));
-          n = nsh + (ndp - nsh) * (t * t * (T(3) - T(2) * t));
+          n = mad_t(ndp - nsh, t * t * (T(3) - T(2) * t), nsh);
         }
         const bool wet = h > T(1e-6);
-        const T inv_h = vdiv(h, h * h + T(1e-3));
-        const T Sf = n * n * fsqrt_t(u * u + v * v) * pow43(inv_h);
+        const T inv_h = vdiv(h, mad_t(h, h, T(1e-3)));
+        const T Sf = n * n * fsqrt_t(mad_t(u, u, v * v)) * pow43(inv_h);
         T dum = T(-kG) * Sf * u;
         T dvm = T(-kG) * Sf * v;
         // overshoot cap with the full timestep
@@ -782,10 +787,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
             const T bb = nh - bh;
             nlo = (bh - (nh - bb)) + (b - bb);
           } else {
-            nh = bh + dhdt * sc[j];
+            nh = mad_t(dhdt, sc[j], bh);
           }
-          T nu = bu + dudt * sc[j];
-          T nv = bv + dvdt * sc[j];
+          T nu = mad_t(dudt, sc[j], bu);
+          T nv = mad_t(dvdt, sc[j], bv);
           if (j == 0) {  // row s+1 is complete after this stage
             if (A.flags & 0x80) {  // E8
               const bool dryc = nh <= A.h_dry;
@@ -810,7 +815,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         if (A.last) {
           // State.cpp:211-222 on the new state; max_i dt*q_i == dt*max_i q_i
           const T cc = fsqrt_t(G * fmax_t(oh, T(0)));
-          const T q = dv<QUICK>(fabs_t(ou) + cc, dx, rdx) + dv<QUICK>(fabs_t(ov) + cc, dy, rdy);
+          const T q = dv2<QUICK>(fabs_t(ou) + cc, dx, rdx, fabs_t(ov) + cc, dy, rdy);
           qmax = (own && q > qmax) ? q : qmax;
           vol += own ? double(oh) + double(ol) : 0.0;
         }
