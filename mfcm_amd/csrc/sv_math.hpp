@@ -105,6 +105,20 @@ template <typename T>
 __device__ __forceinline__ T sy_sign(T x) {
   return x > T(0) ? T(1) : (x < T(0) ? T(-1) : (x != x ? T(0) : x));
 }
+// min(max(x, lo), hi) for lo <= hi: one median instruction in FP32.
+__device__ __forceinline__ float clamp_t(float x, float lo, float hi) {
+  return __builtin_amdgcn_fmed3f(x, lo, hi);
+}
+__device__ __forceinline__ double clamp_t(double x, double lo, double hi) {
+  return __builtin_fmin(__builtin_fmax(x, lo), hi);
+}
+// Sign bits differ.
+__device__ __forceinline__ bool signs_differ(float a, float b) {
+  return (__float_as_int(a) ^ __float_as_int(b)) < 0;
+}
+__device__ __forceinline__ bool signs_differ(double a, double b) {
+  return (__double2hiint(a) ^ __double2hiint(b)) < 0;
+}
 // sycl::min on floating point: y < x ? y : x.
 template <typename T>
 __device__ __forceinline__ T sy_min(T x, T y) { return (y < x) ? y : x; }

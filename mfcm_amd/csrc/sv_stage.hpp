@@ -397,7 +397,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   double vol = 0.0, rate = 0.0;
   T qmax = T(0);
 
-#pragma unroll 2
+#pragma unroll 4
   for (int c = y0 - 1; c <= y1; ++c) {
     const Row<T> Rn = load(c + 2);  // prefetch
     // inputs of row c-1's cell pass: FP32 loads them here (consumed after the
@@ -556,6 +556,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       // bed slope limited to |dz/dx| <= h/dx
       T dudt_bed, dvdt_bed;
       if constexpr (QUICK) {
+        // sycl::sign, not copysign: a negative depth passes the test with a
+        // zero slope, whose sign is zero
         const T zx = (fabs_t(p_zx) > h) ? sy_sign(p_zx) * h : p_zx;
         const T zy = (fabs_t(p_zy) > h) ? sy_sign(p_zy) * h : p_zy;
         dudt_bed = gx * zx;
@@ -589,7 +591,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         T n = nsh;  // mix(a, a, t) == a
         if (nsh != ndp) {
           T t = vdiv(h - dsh, ddp - dsh);
-          t = fmin_t(fmax_t(t, T(0)), T(1));
+          t = clamp_t(t, T(0), T(1));
           n = mad_t(ndp - nsh, t * t * (T(3) - T(2) * t), nsh);
         }
         const bool wet = h > T(1e-6);
@@ -599,8 +601,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         T dvm = T(-kG) * Sf * v;
         // overshoot cap with the full timestep
         const T pu = dum * dtT, pv = dvm * dtT;
-        if (fabs_t(pu) > fabs_t(u) && sy_sign(pu) != sy_sign(u)) dum = div_dt(-u, dtT, rdt);
-        if (fabs_t(pv) > fabs_t(v) && sy_sign(pv) != sy_sign(v)) dvm = div_dt(-v, dtT, rdt);
+        // pu is a multiple of u: where |pu| > |u| both are non-zero and not
+        // NaN, and their signs differ exactly when their sign bits do
+        if (fabs_t(pu) > fabs_t(u) && signs_differ(pu, u)) dum = div_dt(-u, dtT, rdt);
+        if (fabs_t(pv) > fabs_t(v) && signs_differ(pv, v)) dvm = div_dt(-v, dtT, rdt);
         dudt = wet ? dudt + dum : dudt;
         dvdt = wet ? dvdt + dvm : dvdt;
         if constexpr (KOUT) {
@@ -798,8 +802,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
               nv = dryc ? T(0) : nv;
             }
             if (A.flags & 0x100) {  // E9
-              nu = fmin_t(fmax_t(nu, -A.u_max), A.u_max);
-              nv = fmin_t(fmax_t(nv, -A.u_max), A.u_max);
+              nu = clamp_t(nu, -A.u_max, A.u_max);
+              nv = clamp_t(nv, -A.u_max, A.u_max);
             }
             oh = nh; ou = nu; ov = nv; ol = nlo;
           }
