@@ -304,8 +304,12 @@ __device__ __forceinline__ double minmod2(double a, double b) {
   return __builtin_fmax(__builtin_fmin(a, b), __builtin_fmin(__builtin_fmax(a, b), 0.0));
 }
 template <typename T>
+__device__ __forceinline__ T minmod_c(T ql, T qr, T qc) {
+  return minmod2(T(2) * minmod2(ql, qr), T(0.5) * qc);
+}
+template <typename T>
 __device__ __forceinline__ T minmod_q(T ql, T qr, T qc, bool hasl, bool hasr) {
-  const T in = minmod2(T(2) * minmod2(ql, qr), T(0.5) * qc);
+  const T in = minmod_c(ql, qr, qc);
   return hasl ? (hasr ? in : ql) : (hasr ? qr : T(0));
 }
 

@@ -255,6 +255,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   const bool xin = x >= 0 && x < nx;
   const bool own = lane >= kHalo && lane < kWave - kHalo && xin;
   const bool hasW = x > 0, hasE = x < nx - 1;
+  // every lane of the strip has both x neighbours
+  const bool xint = strip * kStrip - kHalo > 0 && strip * kStrip - kHalo + kWave - 1 < nx - 1;
   const bool e7 = (A.flags & 0x40) != 0;
   const bool e2 = (A.flags & 0x02) != 0;
   // does this strip hold the x = 0 or the x = nx face?
@@ -418,10 +420,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     if constexpr (!NANBED) {
       const T qwh = qx(R1.h - W.h), qwu = qx(R1.u - W.u), qwv = qx(R1.v - W.v), qwz = qx(R1.z - W.z);
       const T qeh = from_e(qwh), qeu = from_e(qwu), qev = from_e(qwv), qez = from_e(qwz);
-      shx = minmod_q(qwh, qeh, qx(E.h - W.h), hasW, hasE);
-      sux = minmod_q(qwu, qeu, qx(E.u - W.u), hasW, hasE);
-      svx = minmod_q(qwv, qev, qx(E.v - W.v), hasW, hasE);
-      szx = minmod_q(qwz, qez, qx(E.z - W.z), hasW, hasE);
+      // wave-uniform: no lane of the strip lies on an x edge (the FP64 kernel
+      // keeps the selects: the branch costs it more than it saves)
+      if (sizeof(T) == 4 && xint) {
+        shx = minmod_c(qwh, qeh, qx(E.h - W.h));
+        sux = minmod_c(qwu, qeu, qx(E.u - W.u));
+        svx = minmod_c(qwv, qev, qx(E.v - W.v));
+        szx = minmod_c(qwz, qez, qx(E.z - W.z));
+      } else {
+        shx = minmod_q(qwh, qeh, qx(E.h - W.h), hasW, hasE);
+        sux = minmod_q(qwu, qeu, qx(E.u - W.u), hasW, hasE);
+        svx = minmod_q(qwv, qev, qx(E.v - W.v), hasW, hasE);
+        szx = minmod_q(qwz, qez, qx(E.z - W.z), hasW, hasE);
+      }
       qnh = qy(R2.h - R1.h); qnu = qy(R2.u - R1.u); qnv = qy(R2.v - R1.v); qnz = qy(R2.z - R1.z);
       // south -> north slope; the reference's north -> south slope is its
       // exact negation (Minmod3 commutes with negation)
@@ -431,10 +442,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       if constexpr (QUICK && sizeof(T) == 8) {
         qsh = R1.h - R0.h; qsu = R1.u - R0.u; qsv = R1.v - R0.v; qsz = R1.z - R0.z;
       }
-      const T yh = minmod_q(qsh, qnh, qy(R2.h - R0.h), hasS, hasN);
-      const T yu = minmod_q(qsu, qnu, qy(R2.u - R0.u), hasS, hasN);
-      const T yv = minmod_q(qsv, qnv, qy(R2.v - R0.v), hasS, hasN);
-      const T yz = minmod_q(qsz, qnz, qy(R2.z - R0.z), hasS, hasN);
+      T yh, yu, yv, yz;
+      if (sizeof(T) == 4 && hasS && hasN) {  // wave-uniform: an interior row
+        yh = minmod_c(qsh, qnh, qy(R2.h - R0.h));
+        yu = minmod_c(qsu, qnu, qy(R2.u - R0.u));
+        yv = minmod_c(qsv, qnv, qy(R2.v - R0.v));
+        yz = minmod_c(qsz, qnz, qy(R2.z - R0.z));
+      } else {
+        yh = minmod_q(qsh, qnh, qy(R2.h - R0.h), hasS, hasN);
+        yu = minmod_q(qsu, qnu, qy(R2.u - R0.u), hasS, hasN);
+        yv = minmod_q(qsv, qnv, qy(R2.v - R0.v), hasS, hasN);
+        yz = minmod_q(qsz, qnz, qy(R2.z - R0.z), hasS, hasN);
+      }
       shy = e7 ? yh : -yh; suy = e7 ? yu : -yu; svy = e7 ? yv : -yv; szy = e7 ? yz : -yz;
     } else {
       // general Minmod3 with the reference's NaN rules
