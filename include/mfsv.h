@@ -217,6 +217,18 @@ int mfsv_dudt(mfsv_ctx* ctx, double t_sub, double dt, int want_branches);
 int mfsv_diagnose(mfsv_ctx* ctx, int source, double t_sub, double dt, int what);
 /* end_of_step (Solver.hpp:90-98): STAGE = z_bed + h of U0. */
 int mfsv_update_stage_field(mfsv_ctx* ctx);
+/* SaintVenantSolver::control_number(state_no, dt) (State.cpp:199-229):
+ * max over cells of dt*((|u| + c)/dx + (|v| + c)/dy), c = sqrt(9.81 max(h,0)),
+ * of state 0 (U0), 1..nstages-1 (the intermediate RK rows of the last
+ * timestep) or nstages (the last timestep's result, until it is accepted). */
+int mfsv_control_number(mfsv_ctx* ctx, int state, double dt, double* cn);
+/* SpatialDerivativeOperator<T, Mesh, Cell, Minmod3>::apply<Axis>
+ * (SpatialDerivative.hpp:48-53, 104-122; Minmod3.hpp:34-82) on an FP64 cell
+ * field of the context's mesh: axis 0 = X (west, east), axis 1 = Y with the
+ * reference's orientation (north as the left neighbour).  in and out hold
+ * nx*ny values each. */
+int mfsv_spatial_derivative(mfsv_ctx* ctx, int axis, const double* in, double* out,
+                            int64_t count);
 /* One whole timestep from U0 at local time t_local with timestep dt: every
  * RK stage (RungeKutta.cpp:143-153, update_k + update_y) fused per stage, and
  * the control number of the result (State.cpp:199-229) returned in *cn.

@@ -452,6 +452,53 @@ int compute_volume(mfsv_ctx* c, int which, double* out) {
   return MFSV_OK;
 }
 
+// State.cpp:211-222: per-block maxima of (|u| + c)/dx + (|v| + c)/dy with
+// c = sqrt(9.81 max(h, 0)) (quotients correctly rounded); the caller takes
+// the maximum of the block maxima and multiplies by dt (max_i dt q_i ==
+// dt max_i q_i for dt > 0).
+template <typename T>
+__global__ void k_cn(const T* __restrict__ h, const T* __restrict__ u, const T* __restrict__ v,
+                     int64_t n, T dx, T rdx, T dy, T rdy, double* part) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T c = fsqrt_t(T(kG) * fmax_t(h[i], T(0)));
+    const T q = div_c(fabs_t(u[i]) + c, dx, rdx) + div_c(fabs_t(v[i]) + c, dy, rdy);
+    m = (double(q) > m) ? double(q) : m;
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// SpatialDerivativeOperationKernel with Minmod3 (SpatialDerivative.hpp:104-122,
+// Minmod3.hpp:34-82) on a FP64 cell field: axis 0 takes (west, east) as
+// (left, right), axis 1 the reference's (north, south).  A missing neighbour
+// makes the difference one-sided.
+__global__ void k_slope(const double* __restrict__ f, int64_t nx, int64_t ny, int axis, double d,
+                        double rd, double* __restrict__ out) {
+  const int64_t n = nx * ny;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t x = i % nx, y = i / nx;
+    bool hl, hr;
+    int64_t il, ir;
+    if (axis == 0) {
+      hl = x > 0; hr = x < nx - 1; il = i - 1; ir = i + 1;
+    } else {
+      hl = y < ny - 1; hr = y > 0; il = i + nx; ir = i - nx;
+    }
+    const double uc = f[i];
+    const double ul = hl ? f[il] : uc, ur = hr ? f[ir] : uc;
+    out[i] = minmod3(ul, hl, uc, hr, ur, d, rd);
+  }
+}
+
 int set_ctrl(mfsv_ctx* c, double t_local, double dt, int flip, int set_time) {
   hipLaunchKernelGGL(k_set_ctrl, dim3(1), dim3(64), 0, c->stream, tp<Ctrl>(c->ctrl), t_local, dt,
                      flip, set_time);
@@ -855,6 +902,66 @@ int mfsv_diagnose(mfsv_ctx* c, int source, double t_sub, double dt, int what) {
   // continue after this call)
   if ((rc = set_ctrl(c, keep_t, keep_dt, 0, 1))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFSV_OK;
+}
+
+int mfsv_control_number(mfsv_ctx* c, int state, double dt, double* cn) {
+  if (!c || !cn || !(dt > 0) || state < 0 || state > c->nstages) return MFSV_ERR_ARG;
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  if (!c->U[0][0].p) return fail(c, MFSV_ERR_STATE, "the state must be uploaded");
+  int rc;
+  if ((rc = read_ctrl(c))) return rc;
+  c->cur = c->hctrl->cur;
+  const Buf* f;  // U_0, the rows U_1..U_{n-1}, U_n (the result buffer)
+  if (state == 0) f = c->U[c->cur];
+  else if (state == c->nstages) f = c->U[c->cur ^ 1];
+  else f = c->acc[state];
+  if (!f[0].p) return fail(c, MFSV_ERR_STATE, "that state holds no values");
+  if ((rc = alloc(c, c->volp, sizeof(double) * (kVolBlocks + 1)))) return rc;
+  double* part = tp<double>(c->volp);
+  if (c->prec == MFSV_F64)
+    hipLaunchKernelGGL(k_cn<double>, dim3(kVolBlocks), dim3(256), 0, c->stream, tp<double>(f[0]),
+                       tp<double>(f[1]), tp<double>(f[2]), c->nc, c->dx, 1.0 / c->dx, c->dy,
+                       1.0 / c->dy, part);
+  else
+    hipLaunchKernelGGL(k_cn<float>, dim3(kVolBlocks), dim3(256), 0, c->stream, tp<float>(f[0]),
+                       tp<float>(f[1]), tp<float>(f[2]), c->nc, float(c->dx), float(1.0 / c->dx),
+                       float(c->dy), float(1.0 / c->dy), part);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> hp(kVolBlocks);
+  HIPCHK(c, hipMemcpyAsync(hp.data(), part, sizeof(double) * kVolBlocks, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double m = 0.0;
+  for (double x : hp) m = x > m ? x : m;
+  *cn = dt * m;
+  return MFSV_OK;
+}
+
+int mfsv_spatial_derivative(mfsv_ctx* c, int axis, const double* in, double* out, int64_t count) {
+  if (!c || !in || !out || (axis != 0 && axis != 1)) return MFSV_ERR_ARG;
+  if (count != c->nc) return fail(c, MFSV_ERR_ARG, "count must equal nx*ny");
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, MFSV_ERR_HIP, "hipSetDevice");
+  Buf a, b;
+  int rc;
+  if ((rc = alloc(c, a, 8 * c->nc)) || (rc = alloc(c, b, 8 * c->nc))) {
+    release(a); release(b);
+    return rc;
+  }
+  const double d = axis == 0 ? c->dx : c->dy;
+  const unsigned blocks = (unsigned)std::min<int64_t>((c->nc + 255) / 256, 4096);
+  hipError_t e = hipMemcpyAsync(a.p, in, 8 * c->nc, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_slope, dim3(blocks), dim3(256), 0, c->stream, tp<double>(a), c->nx, c->ny,
+                       axis, d, 1.0 / d, tp<double>(b));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, b.p, 8 * c->nc, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  c->bytes -= (int64_t)(a.bytes + b.bytes);
+  release(a);
+  release(b);
+  if (e != hipSuccess) return fail(c, MFSV_ERR_HIP, std::string("spatial derivative: ") + hipGetErrorString(e));
   return MFSV_OK;
 }
 

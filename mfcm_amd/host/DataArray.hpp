@@ -150,4 +150,22 @@ class Field {
   int64_t nx_;
 };
 
+// SpatialDerivativeOperator<T, Mesh, Cell, Minmod3> (SpatialDerivative/
+// SpatialDerivative.hpp:48-53, 104-122): the Minmod3 slope of a cell field
+// along X (west to east) or Y (the reference's orientation: north as the
+// left neighbour, so the result is the slope toward decreasing y), evaluated
+// on the context's device.
+enum class SpatialDerivativeAxis { X = 0, Y = 1 };
+
+class SpatialDerivativeOperator {
+ public:
+  template <SpatialDerivativeAxis A>
+  static void apply(const Context& ctx, const Field<double>& in, Field<double>& out) {
+    if (in.size() != out.size()) throw std::logic_error("SpatialDerivativeOperator: field sizes differ");
+    check(mfsv_spatial_derivative(ctx.get(), int(A), in.data().data(), out.data().data(),
+                                  int64_t(in.size())),
+          ctx.get(), "mfsv_spatial_derivative");
+  }
+};
+
 }  // namespace mfsv

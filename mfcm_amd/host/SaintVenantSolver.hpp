@@ -227,8 +227,16 @@ class SaintVenantSolver {
     cn_ = cn;
     open_ = false;
   }
-  // State.cpp:199-229 on the result of the last timestep.
-  double control_number(size_t /*state*/, TT /*dt*/) const { return cn_; }
+  // State.cpp:199-229.  The fused timestep returns the result's control
+  // number with it; other states (and other timesteps) are evaluated on the
+  // device.
+  double control_number(size_t state, TT dt) const {
+    if (state == nstages_ && double(dt) == double(dt_) && !open_) return cn_;
+    double cn = 0.0;
+    check(mfsv_control_number(ctx_->get(), int(state), double(dt), &cn), ctx_->get(),
+          "mfsv_control_number");
+    return cn;
+  }
   void accept() { check(mfsv_accept(ctx_->get()), ctx_->get(), "mfsv_accept"); }
 
   // Solver.cpp:119-129: boundary endpoints for [time_now, time_now + step).

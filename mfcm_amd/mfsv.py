@@ -118,6 +118,8 @@ SIGNATURES = {
     "mfsv_dudt": (_I, [_P, _D, _D, _I]),
     "mfsv_diagnose": (_I, [_P, _I, _D, _D, _I]),
     "mfsv_update_stage_field": (_I, [_P]),
+    "mfsv_control_number": (_I, [_P, _I, _D, C.POINTER(C.c_double)]),
+    "mfsv_spatial_derivative": (_I, [_P, _I, _P, _P, _I64]),
     "mfsv_timestep": (_I, [_P, _D, _D, C.POINTER(C.c_double)]),
     "mfsv_accept": (_I, [_P]),
     "mfsv_volume": (_I, [_P, C.POINTER(C.c_double)]),
@@ -396,6 +398,19 @@ class Catchment:
             out["dzdx_bed"] = self.download("dzdx_bed")
             out["dzdy_bed"] = self.download("dzdy_bed")
         return out
+
+    def control_number(self, state, dt):
+        """State.cpp:199-229 on state 0 (U0), 1..n-1 (RK rows) or n (result)."""
+        cn = C.c_double()
+        self._check(lib().mfsv_control_number(self._h, state, dt, C.byref(cn)))
+        return cn.value
+
+    def spatial_derivative(self, field, axis):
+        """Minmod3 slope of an FP64 cell field (axis 0: x; 1: the reference's y)."""
+        a = np.ascontiguousarray(field, dtype=np.float64).ravel()
+        out = np.empty_like(a)
+        self._check(lib().mfsv_spatial_derivative(self._h, axis, _ptr(a), _ptr(out), a.size))
+        return out.reshape(self.ny, self.nx)
 
     def stage_field(self):
         """z_bed + h of U0 (the solver's stage field)."""

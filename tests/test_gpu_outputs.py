@@ -136,3 +136,52 @@ def test_last_stage_outputs_need_a_timestep(gpu_lib):
     with pytest.raises(MfsvError):
         cat.diagnose(1)
     cat.close()
+
+
+@pytest.mark.parametrize("nx,ny,dx,dy,seed", [(64, 48, 5.0, 3.0, 11), (70, 33, 2.0, 4.0, 12),
+                                              (1, 9, 5.0, 5.0, 13), (13, 1, 3.0, 3.0, 14)])
+def test_spatial_derivative_matches_oracle(gpu_lib, nx, ny, dx, dy, seed):
+    """SpatialDerivativeOperator with Minmod3 on a field with coded-out (NaN)
+    cells, along X and along the reference's Y: identical to the oracle's
+    slope passes, NaN where the oracle has NaN."""
+    from oracle import oracle as O
+    from mfcm_amd.mfsv import Catchment
+    _say(f"spatial derivative nx={nx} ny={ny} dx={dx} dy={dy}")
+    rng = np.random.default_rng(seed)
+    f = rng.normal(0.0, 1.0, (ny, nx))
+    f[rng.uniform(size=(ny, nx)) < 0.1] = np.nan
+    f[rng.uniform(size=(ny, nx)) < 0.1] = 0.25        # ties
+    sc = S.Scenario("sd", nx, ny, dx, dy, np.zeros((ny, nx)), np.zeros((ny, nx)),
+                    np.zeros((ny, nx)), np.zeros((ny, nx)), 0.03)
+    cat = Catchment(sc)
+    gx = cat.spatial_derivative(f, 0)
+    gy = cat.spatial_derivative(f, 1)
+    np.testing.assert_array_equal(gx, O.slope_x(f, dx))
+    np.testing.assert_array_equal(gy, O.slope_y(f, dy, 0))
+    cat.close()
+
+
+@pytest.mark.parametrize("scheme", ["heun2", "classic"])
+def test_control_number_of_each_state(gpu_lib, scheme):
+    """control_number(state, dt) of U0, the intermediate RK rows and the
+    result of one timestep, against the oracle's after the same timestep;
+    the result's equals the value the fused timestep returned."""
+    from oracle import oracle as O
+    from mfcm_amd.mfsv import Catchment
+    sc = S.dam_break_bump(48, 40)
+    sc.scheme = scheme
+    sim = O.sim_from_scenario(sc)
+    sim.start_new_step(0.0)
+    dt = 0.6
+    sim.do_timestep(0.0, dt)
+    cat = Catchment(sc)
+    cat.start_new_step(0.0)
+    cn_fused = cat.control_timestep(0.0, dt)
+    n = cat.nstages
+    for s in range(n + 1):
+        g = cat.control_number(s, dt)
+        o = sim.control_number(s, dt)
+        _say(f"  {scheme} state {s}: {g!r} vs {o!r}")
+        assert g == pytest.approx(o, rel=1e-14), (s, g, o)
+    assert cn_fused == pytest.approx(cat.control_number(n, dt), rel=1e-15)
+    cat.close()
