@@ -23,29 +23,47 @@ constexpr int kCtrlWaves = kCtrlThreads / kWave;
 constexpr int kCtrlVals = 2 + kMaxStages;  // volume, control number, stage rates
 
 // Volume and per-stage source-rate sums and the control-number maximum of the
-// tile partials, in one pass: thread t takes tiles t, t + 1024, ...
-// (independent accumulators), then a butterfly inside each wave and a
-// fixed-order pass over the waves.  Deterministic for a given tile count.
+// tile partials, in one pass: thread t takes tiles t, t + 1024, ... in that
+// order, issuing the loads of four tiles before summing them (the pass is
+// latency-bound), then a butterfly inside each wave and a fixed-order pass
+// over the waves.  Deterministic for a given tile count.
 // out[0] = volume partial sum, out[1] = max control number, out[2 + s] =
 // stage s rate sum.
 __device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double* out) {
+  constexpr int kU = 4;
   double acc[kCtrlVals];
 #pragma unroll
   for (int q = 0; q < kCtrlVals; ++q) acc[q] = 0.0;
-  for (int64_t k = threadIdx.x; k < A.ntiles; k += blockDim.x) {
-    acc[0] += A.part_vol[k];
-    const double cn = A.part_vol[A.ntiles + k];
-    acc[1] = (cn > acc[1]) ? cn : acc[1];
+  const int64_t nt = A.ntiles;
+  for (int64_t k0 = threadIdx.x; k0 < nt; k0 += int64_t(kU) * blockDim.x) {
+    double v[kU][kCtrlVals];
 #pragma unroll
-    for (int s = 0; s < kMaxStages; ++s)
-      if (s < A.nstages) acc[2 + s] += A.part_rate[(int64_t)s * A.ntiles + k];
+    for (int u = 0; u < kU; ++u) {
+      const int64_t k = k0 + int64_t(u) * blockDim.x;
+      const bool in = k < nt;
+      v[u][0] = in ? A.part_vol[k] : 0.0;
+      v[u][1] = in ? A.part_vol[nt + k] : 0.0;
+#pragma unroll
+      for (int s = 0; s < kMaxStages; ++s)
+        v[u][2 + s] = (in && s < A.nstages) ? A.part_rate[int64_t(s) * nt + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      acc[0] += v[u][0];
+      acc[1] = (v[u][1] > acc[1]) ? v[u][1] : acc[1];
+#pragma unroll
+      for (int s = 0; s < kMaxStages; ++s) acc[2 + s] += v[u][2 + s];
+    }
   }
+  const int nv = 2 + A.nstages;  // values in use
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
     for (int q = 0; q < kCtrlVals; ++q) {
-      const double o = __shfl_xor(acc[q], off, kWave);
-      acc[q] = (q == 1) ? ((o > acc[q]) ? o : acc[q]) : acc[q] + o;
+      if (q < nv) {
+        const double o = __shfl_xor(acc[q], off, kWave);
+        acc[q] = (q == 1) ? ((o > acc[q]) ? o : acc[q]) : acc[q] + o;
+      }
     }
   }
   const int w = threadIdx.x >> 6;
@@ -53,17 +71,22 @@ __device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double*
 #pragma unroll
     for (int q = 0; q < kCtrlVals; ++q) sh[w * kCtrlVals + q] = acc[q];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = (int)(blockDim.x >> 6);
+  // thread q < kCtrlVals folds value q over the waves in wave order, all
+  // its LDS reads issued before the first addition
+  if (threadIdx.x < kCtrlVals) {
+    const int q = threadIdx.x;
+    double v[kCtrlWaves];
 #pragma unroll
-    for (int q = 0; q < kCtrlVals; ++q) {
-      double t = 0.0;
-      for (int i = 0; i < nw; ++i) {
-        const double v = sh[i * kCtrlVals + q];
-        t = (q == 1) ? ((v > t) ? v : t) : t + v;
-      }
-      out[q] = t;
-    }
+    for (int i = 0; i < kCtrlWaves; ++i) v[i] = (i * kWave < (int)blockDim.x) ? sh[i * kCtrlVals + q] : 0.0;
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kCtrlWaves; ++i) t = (q == 1) ? ((v[i] > t) ? v[i] : t) : t + v[i];
+    sh[kCtrlWaves * kCtrlVals + q] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < kCtrlVals; ++q) out[q] = sh[kCtrlWaves * kCtrlVals + q];
   }
 }
 
@@ -71,21 +94,29 @@ __device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double*
 // landing rules and outer-step transitions (TemporalScheme.hpp:112-149).
 __device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
   Ctrl* C = A.ctrl;
-  if (C->done) return;
+  // thread 0 reads the control block while the partial sums load, works on
+  // its copy and writes it back once
+  Ctrl L;
+  if (threadIdx.x == 0) L = *C;
   double sums[kCtrlVals];
   tile_sums(A, sh, sums);
   if (threadIdx.x != 0) return;
+  if (L.done) return;
   const double vol = sums[0] * A.area;
   double rate = 0.0;
 #pragma unroll
   for (int s = 0; s < kMaxStages; ++s)
     if (s < A.nstages) rate += A.b[s] * sums[2 + s];
   const double cn_raw = sums[1];
-  C->cn_bits = 0ull;
-  C->cn = cn_raw;
-  if (A.mode == 1) { C->volume = vol; return; }
-  const double cn = (C->cn_target == 1.0) ? cn_raw : cn_raw / C->cn_target;
-  const double dt = C->dt;
+  L.cn_bits = 0ull;
+  L.cn = cn_raw;
+  if (A.mode == 1) {
+    L.volume = vol;
+    *C = L;
+    return;
+  }
+  const double cn = (L.cn_target == 1.0) ? cn_raw : cn_raw / L.cn_target;
+  const double dt = L.dt;
   bool repeat;
   double ndt;
   if (cn > 1.0) {
@@ -96,38 +127,39 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
   } else {
     repeat = false; ndt = dt / cn;
   }
-  C->repeat = repeat ? 1 : 0;
+  L.repeat = repeat ? 1 : 0;
   if (repeat) {
-    C->n_rej++;
+    L.n_rej++;
   } else {
-    C->n_acc++;
-    C->t_local += dt;
-    C->cur ^= 1;
-    C->volume = vol;
-    C->srcvol += dt * rate * A.area;
+    L.n_acc++;
+    L.t_local += dt;
+    L.cur ^= 1;
+    L.volume = vol;
+    L.srcvol += dt * rate * A.area;
   }
-  C->dt = ndt;
-  const double sd = C->step_dur, tl = C->t_local;
+  L.dt = ndt;
+  const double sd = L.step_dur, tl = L.t_local;
   if (tl >= sd) {
-    C->step++;
-    C->t_local = 0.0;
+    L.step++;
+    L.t_local = 0.0;
     // TemporalScheme.hpp:254: time_now = start_time + i * step_duration
-    C->gstep = C->start_time + double(C->step) * sd;
-    if (C->step >= C->nsteps) {
-      C->done = 1;
-    } else if (A.forcing && C->step < A.nforcing) {
-      const double* f = A.forcing + 4 * C->step;
-      C->q0 = f[0]; C->q1 = f[1]; C->hb0 = f[2]; C->hb1 = f[3];
+    L.gstep = L.start_time + double(L.step) * sd;
+    if (L.step >= L.nsteps) {
+      L.done = 1;
+    } else if (A.forcing && L.step < A.nforcing) {
+      const double* f = A.forcing + 4 * L.step;
+      L.q0 = f[0]; L.q1 = f[1]; L.hb0 = f[2]; L.hb1 = f[3];
     }
   } else if (tl + ndt > sd) {
-    C->dt = sd - tl;
+    L.dt = sd - tl;
   } else if (tl + 2.0 * ndt > sd) {
-    C->dt = (sd - tl) * 0.4;
+    L.dt = (sd - tl) * 0.4;
   }
+  *C = L;
 }
 
 __global__ __launch_bounds__(kCtrlThreads) void k_ctrl(const CtrlArgs A) {
-  __shared__ double sh[kCtrlWaves * kCtrlVals];
+  __shared__ double sh[(kCtrlWaves + 1) * kCtrlVals];
   ctrl_body(A, sh);
 }
 
@@ -135,7 +167,7 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl(const CtrlArgs A) {
 // memory so that captured graphs survive changes of membership.
 __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_batch(const CtrlArgs* __restrict__ args,
                                                              const int* __restrict__ nmembers_p) {
-  __shared__ double sh[kCtrlWaves * kCtrlVals];
+  __shared__ double sh[(kCtrlWaves + 1) * kCtrlVals];
   if ((int)blockIdx.x >= *nmembers_p) return;
   ctrl_body(args[blockIdx.x], sh);
 }
