@@ -274,8 +274,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   const T* const xu = A.in_u ? A.in_u : (cur ? A.Uu[1] : A.Uu[0]);
   const T* const xv = A.in_v ? A.in_v : (cur ? A.Uv[1] : A.Uv[0]);
   const T* __restrict__ zb = A.zb;
-  const T* __restrict__ nf = A.nfield;
-  const bool has_n = nf != nullptr;
+  const bool has_n = A.nfield != nullptr;
+  // FP32: without a roughness field the row's n load reads the bed instead
+  // (a valid address; the value is not used): no branch around the load, so
+  // its wait does not also wait for the row prefetch
+  const T* __restrict__ nf = has_n ? A.nfield : A.zb;
   const T dx = A.dx, dy = A.dy, rdx = A.rdx, rdy = A.rdy;
   const T hdx = QUICK ? T(0.5) : A.hdx, hdy = QUICK ? T(0.5) : A.hdy;
   const T gx = QUICK ? T(-kG) * rdx : T(-kG), gy = QUICK ? T(-kG) * rdy : T(-kG);
@@ -365,7 +368,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     T n;
     T bh[NB], bu[NB], bv[NB];
     T lo;  // FP32 form: low-order depth of the final row's base
+    // FP32 form: per-cell boundary endpoints (discharge, head)
+    T q0, q1, hb0, hb1;
   };
+  // FP32: the per-cell boundary endpoints are loaded with the other cell-pass
+  // inputs, ahead of the row prefetch and without a branch (a field that is
+  // off reads the bed; its value is not used), so that no wait in the cell
+  // pass also waits for the prefetch.  FP64 loads them where they are used.
+  constexpr bool kPreBnd = sizeof(T) == 4;
+  const T* const q0p = A.discharge == 2 ? A.q0f : zb;
+  const T* const q1p = A.discharge == 2 ? A.q1f : zb;
+  const T* const hb0p = A.head == 2 ? A.hb0f : zb;
+  const T* const hb1p = A.head == 2 ? A.hb1f : zb;
   T* const hl_in = (sizeof(T) == 4 && !KOUT) ? (A.stage == 0 ? (cur ? A.Uhl[1] : A.Uhl[0])
                                                               : (cur ? A.Uhl[0] : A.Uhl[1]))
                                              : nullptr;
@@ -373,9 +387,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   auto load_extra = [&](int r) {
     Extra e;
     const uint32_t o = roff(r);
-    e.n = has_n ? ld(nf, o) : A.n_sh;
+    if constexpr (sizeof(T) == 4) e.n = ld(nf, o);
+    else e.n = has_n ? ld(nf, o) : A.n_sh;
     e.lo = T(0);
     if constexpr (sizeof(T) == 4 && !KOUT) e.lo = ld(hl_in, o);
+    if constexpr (kPreBnd) {
+      e.q0 = ld(q0p, o); e.q1 = ld(q1p, o); e.hb0 = ld(hb0p, o); e.hb1 = ld(hb1p, o);
+    } else {
+      e.q0 = e.q1 = e.hb0 = e.hb1 = T(0);
+    }
 #pragma unroll
     for (int j = 0; j < NB; ++j) { e.bh[j] = T(0); e.bu[j] = T(0); e.bv[j] = T(0); }
     if constexpr (base_ld) {
@@ -401,14 +421,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
 
 #pragma unroll 4
   for (int c = y0 - 1; c <= y1; ++c) {
-    const Row<T> Rn = load(c + 2);  // prefetch
     // inputs of row c-1's cell pass: FP32 loads them here (consumed after the
-    // slopes and faces; fewer live registers), FP64 one row ahead (its
-    // occupancy is lower, so memory latency needs the longer lead)
+    // slopes and faces; fewer live registers) and before the row prefetch,
+    // so that waiting for them leaves the prefetch in flight; FP64 one row
+    // ahead (its occupancy is lower, so memory latency needs the longer lead)
     Extra X0;
-    if constexpr (sizeof(T) == 4) {
-      X0 = load_extra(c - 1);
-    } else {
+    if constexpr (sizeof(T) == 4) X0 = load_extra(c - 1);
+    const Row<T> Rn = load(c + 2);  // prefetch
+    if constexpr (sizeof(T) == 8) {
       X0 = Xn;
       Xn = load_extra(c);
     }
@@ -715,7 +735,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       // DischargeBoundarySourceTerm.cpp:43-67 (E1 flips the guard)
       if (A.discharge || storm) {
         T tq0, tq1;
-        if (A.discharge == 2) { tq0 = ld(A.q0f, wo); tq1 = ld(A.q1f, wo); }
+        if (A.discharge == 2) {
+          if constexpr (kPreBnd) { tq0 = X0.q0; tq1 = X0.q1; }
+          else { tq0 = ld(A.q0f, wo); tq1 = ld(A.q1f, wo); }
+        }
         else if (A.discharge == 1) { tq0 = T(q0s); tq1 = T(q1s); }
         else { tq0 = T(0); tq1 = T(0); }
         if (storm) {
@@ -741,7 +764,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       // HeadBoundarySourceTerm.cpp:43-66 (overwrites dh)
       if (A.head) {
         T h0, h1;
-        if (A.head == 2) { h0 = ld(A.hb0f, wo); h1 = ld(A.hb1f, wo); }
+        if (A.head == 2) {
+          if constexpr (kPreBnd) { h0 = X0.hb0; h1 = X0.hb1; }
+          else { h0 = ld(A.hb0f, wo); h1 = ld(A.hb1f, wo); }
+        }
         else {
           const bool in = x >= A.hx0 && x < A.hx1;
           h0 = in ? T(hb0s) : Num<T>::nan();
