@@ -119,6 +119,16 @@ __device__ __forceinline__ bool signs_differ(float a, float b) {
 __device__ __forceinline__ bool signs_differ(double a, double b) {
   return (__double2hiint(a) ^ __double2hiint(b)) < 0;
 }
+// The bed slope limit of TemporalDerivativeKernel.cpp:72-80 in FP32,
+// (|x| > h) ? sycl::sign(x) * h : x, without the sign function: for x != 0
+// the product is h with x's sign bit applied, and with hp = max(h, 0) the
+// test passes exactly where the reference's does for x != 0 (a negative
+// depth passes it for every non-zero slope); a zero x is returned as it is
+// where the reference returns a zero of either sign.
+__device__ __forceinline__ float bed_limit(float x, float h, float hp) {
+  const float s = __int_as_float(__float_as_int(h) ^ (__float_as_int(x) & int(0x80000000u)));
+  return (fabs_t(x) > hp) ? s : x;
+}
 // sycl::min on floating point: y < x ? y : x.
 template <typename T>
 __device__ __forceinline__ T sy_min(T x, T y) { return (y < x) ? y : x; }

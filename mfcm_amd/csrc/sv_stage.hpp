@@ -282,6 +282,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   const T dx = A.dx, dy = A.dy, rdx = A.rdx, rdy = A.rdy;
   const T hdx = QUICK ? T(0.5) : A.hdx, hdy = QUICK ? T(0.5) : A.hdy;
   const T gx = QUICK ? T(-kG) * rdx : T(-kG), gy = QUICK ? T(-kG) * rdy : T(-kG);
+  // FP32 difference form: the y slopes stay south -> north (the reference's
+  // run north -> south unless E7, their exact negation) and the sign rides on
+  // the y half spacing and the y pressure and bed factors instead (exact: a
+  // product changes sign with either factor)
+  constexpr bool kFoldY = sizeof(T) == 4 && QUICK && !NANBED && !EXT && !KOUT;
+  const T hdy_s = (kFoldY && !e7) ? -hdy : hdy;
+  const T gy_s = (kFoldY && !e7) ? -gy : gy;
   const T G = T(kG);
 
   // RK coefficients of this stage: (T)(a * dt) as State::operator*(ValueType).
@@ -474,7 +481,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         yv = minmod_q(qsv, qnv, qy(R2.v - R0.v), hasS, hasN);
         yz = minmod_q(qsz, qnz, qy(R2.z - R0.z), hasS, hasN);
       }
-      shy = e7 ? yh : -yh; suy = e7 ? yu : -yu; svy = e7 ? yv : -yv; szy = e7 ? yz : -yz;
+      if constexpr (kFoldY) {
+        shy = yh; suy = yu; svy = yv; szy = yz;
+      } else {
+        shy = e7 ? yh : -yh; suy = e7 ? yu : -yu; svy = e7 ? yv : -yv; szy = e7 ? yz : -yz;
+      }
     } else {
       // general Minmod3 with the reference's NaN rules
       const Row<T> Wc = hasW ? W : R1, Ec = hasE ? E : R1;
@@ -538,16 +549,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       Flux<T> F;
       if (c == ny) {  // north edge of the top row
         const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
-        if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(L, wall_ghost(L), hdy, dy, rdy, &br);
+        if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(L, wall_ghost(L), hdy_s, dy, rdy, &br);
         else F = face_flux<T, kRel>(L, L, 1, hdy, dy, rdy, &br);
       } else {
         const Side<T> R = {R1.h, R1.v, R1.u, R1.z, shy, svy, suy, szy};
         if (c == 0) {
-          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(wall_ghost(R), R, hdy, dy, rdy, &br);
+          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(wall_ghost(R), R, hdy_s, dy, rdy, &br);
           else F = face_flux<T, kRel>(R, R, -1, hdy, dy, rdy, &br);
         } else {
           const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
-          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(L, R, hdy, dy, rdy, &br);
+          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(L, R, hdy_s, dy, rdy, &br);
           else F = face_flux<T, kRel>(L, R, 0, hdy, dy, rdy, &br);
         }
       }
@@ -591,16 +602,23 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       T dudt = dv2<QUICK>(Xw0.u - Xe0.u, dx, rdx, Fs0.u - Fs1.u, dy, rdy);
       T dvdt = dv2<QUICK>(Xw0.v - Xe0.v, dx, rdx, Fs0.v - Fs1.v, dy, rdy);
       dudt = mad_t(p_hx, gx, dudt);
-      dvdt = mad_t(p_hy, gy, dvdt);
+      dvdt = mad_t(p_hy, gy_s, dvdt);
       // bed slope limited to |dz/dx| <= h/dx
       T dudt_bed, dvdt_bed;
       if constexpr (QUICK) {
-        // sycl::sign, not copysign: a negative depth passes the test with a
-        // zero slope, whose sign is zero
-        const T zx = (fabs_t(p_zx) > h) ? sy_sign(p_zx) * h : p_zx;
-        const T zy = (fabs_t(p_zy) > h) ? sy_sign(p_zy) * h : p_zy;
+        T zx, zy;
+        if constexpr (sizeof(T) == 4) {
+          const T hp = fmax_t(h, T(0));
+          zx = bed_limit(p_zx, h, hp);
+          zy = bed_limit(p_zy, h, hp);
+        } else {
+          // sycl::sign, not copysign: a negative depth passes the test with a
+          // zero slope, whose sign is zero
+          zx = (fabs_t(p_zx) > h) ? sy_sign(p_zx) * h : p_zx;
+          zy = (fabs_t(p_zy) > h) ? sy_sign(p_zy) * h : p_zy;
+        }
         dudt_bed = gx * zx;
-        dvdt_bed = gy * zy;
+        dvdt_bed = gy_s * zy;
       } else {
         T dzdx = p_zx;
         const T hx_ = div_c(h, dx, rdx);
