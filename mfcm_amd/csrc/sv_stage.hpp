@@ -826,7 +826,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           dhdt = (!(h0 != h0) && !(h1 != h1)) ? div_dt(target - h, dtT, rdt) : dhdt;
         }
       }
-      rate += own ? double(dhdt) : 0.0;
+      rate += double(dhdt);  // lanes that own no cell are dropped after the march
       if constexpr (KOUT) {
         if (own) {
           st(A.k_h, wo, dhdt);
@@ -883,8 +883,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           // State.cpp:211-222 on the new state; max_i dt*q_i == dt*max_i q_i
           const T cc = fsqrt_t(G * fmax_t(oh, T(0)));
           const T q = dv2<QUICK>(fabs_t(ou) + cc, dx, rdx, fabs_t(ov) + cc, dy, rdy);
-          qmax = (own && q > qmax) ? q : qmax;
-          vol += own ? double(oh) + double(ol) : 0.0;
+          qmax = (q > qmax) ? q : qmax;
+          vol += double(oh) + double(ol);
         }
       }
     }
@@ -896,7 +896,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     Xw0 = Xw1;
   }
 
-  // wave reductions (fixed butterfly order -> deterministic per tile)
+  // the lanes that own a cell, then wave reductions (fixed butterfly order ->
+  // deterministic per tile)
+  if (!own) { rate = 0.0; vol = 0.0; qmax = T(0); }
   double cnmax = dt * double(qmax);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
