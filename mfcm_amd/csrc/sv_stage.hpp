@@ -378,18 +378,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     // FP32 form: per-cell boundary endpoints (discharge, head)
     T q0, q1, hb0, hb1;
   };
-  // FP32: the per-cell boundary endpoints are loaded with the other cell-pass
-  // inputs, ahead of the row prefetch and without a branch (a field that is
-  // off reads the bed; its value is not used), so that no wait in the cell
-  // pass also waits for the prefetch.  FP64 loads them where they are used.
-  constexpr bool kPreBnd = sizeof(T) == 4;
-  const T* const q0p = A.discharge == 2 ? A.q0f : zb;
-  const T* const q1p = A.discharge == 2 ? A.q1f : zb;
-  const T* const hb0p = A.head == 2 ? A.hb0f : zb;
-  const T* const hb1p = A.head == 2 ? A.hb1f : zb;
   T* const hl_in = (sizeof(T) == 4 && !KOUT) ? (A.stage == 0 ? (cur ? A.Uhl[1] : A.Uhl[0])
                                                               : (cur ? A.Uhl[0] : A.Uhl[1]))
                                              : nullptr;
+  // FP32: the per-cell boundary endpoints are loaded with the other cell-pass
+  // inputs, ahead of the row prefetch and without a branch, so that no wait
+  // in the cell pass also waits for the prefetch.  A field that is off reads
+  // the words the row's low-order depth load has just fetched (the bed where
+  // there is none); its value is not used.  FP64 loads them where they are
+  // used.
+  constexpr bool kPreBnd = sizeof(T) == 4;
+  const T* const fb = (sizeof(T) == 4 && !KOUT) ? hl_in : zb;
+  const T* const q0p = A.discharge == 2 ? A.q0f : fb;
+  const T* const q1p = A.discharge == 2 ? A.q1f : fb;
+  const T* const hb0p = A.head == 2 ? A.hb0f : fb;
+  const T* const hb1p = A.head == 2 ? A.hb1f : fb;
   constexpr bool base_ld = BASE;
   auto load_extra = [&](int r) {
     Extra e;
