@@ -114,7 +114,7 @@ class TileTuner:
                    candidates=CANDIDATES) -> int:
         """Time a shared tile height for a batch: scratch copies of the members
         run `samples` timesteps per candidate (the batch's event-timed run).
-        The members advance while they are timed, so a candidate is judged on
+        The members advance while they are timed, so a candidate is timed on
         a state a little later than the previous one; the scratch states are
         discarded."""
         from .batch import Batch
