@@ -417,7 +417,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     return e;
   };
   Extra Xn;
-  if constexpr (sizeof(T) == 8) Xn = load_extra(y0 - 1);
+  // the cell-pass inputs of the rows above the tile are never used: those
+  // iterations load row y0's (needed a row or two later, so the loads are not
+  // wasted on HBM); likewise the last iteration's prefetch reloads row y1 + 1
+  if constexpr (sizeof(T) == 8) Xn = load_extra(y0);
   // south slope quantities of the current row (rows y0-2 and y0-1 first)
   T qsh = qy(R1.h - R0.h), qsu = qy(R1.u - R0.u), qsv = qy(R1.v - R0.v), qsz = qy(R1.z - R0.z);
   // slopes of row c-1 that the cell pass and the y-face still need
@@ -436,11 +439,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     // so that waiting for them leaves the prefetch in flight; FP64 one row
     // ahead (its occupancy is lower, so memory latency needs the longer lead)
     Extra X0;
-    if constexpr (sizeof(T) == 4) X0 = load_extra(c - 1);
-    const Row<T> Rn = load(c + 2);  // prefetch
+    if constexpr (sizeof(T) == 4) X0 = load_extra(c - 1 < y0 ? y0 : c - 1);
+    const Row<T> Rn = load(c + 2 > y1 + 1 ? y1 + 1 : c + 2);  // prefetch
     if constexpr (sizeof(T) == 8) {
       X0 = Xn;
-      Xn = load_extra(c);
+      Xn = load_extra(c < y0 ? y0 : c);
     }
     const Row<T> W = from_w(R1), E = from_e(R1);
     const bool hasN = c + 1 < ny, hasS = c > 0;
