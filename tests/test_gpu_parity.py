@@ -255,3 +255,112 @@ def test_run_further_sources_matches_oracle(gpu_lib, graph):
     resid = st["volume"] - st["volume0"] - st["source_volume"]
     assert abs(resid) <= 1e-9 * max(st["volume"], 1.0), resid
     cat.close()
+
+
+@pytest.mark.parametrize("prec_name", ["f64", "f32"])
+@pytest.mark.parametrize("storm", [False, True])
+def test_per_cell_boundaries_equal_uniform(gpu_lib, prec_name, storm):
+    """Per-cell discharge and head endpoints (what the configuration path's
+    selections upload) give the same states, bit for bit, as the uniform
+    rain series and the head band they restate: the per-cell values are the
+    same numbers rounded the same way, so any difference is a kernel reading
+    the wrong word (the FP32 form loads these fields ahead of the row
+    prefetch, from a stand-in address when a field is off)."""
+    import ctypes as C
+    from mfcm_amd import mfsv as M
+    sc = S.hillslope_rain(seed=6, n=70, dx=5.0, prewet=0.002, t_end=180.0)
+    sc.ny = 45
+    for a in ("zb", "h0", "u0", "v0"):
+        setattr(sc, a, getattr(sc, a)[:45].copy())
+    sc.head_cols = 3
+    if storm:
+        sc.storm = S.drifting_storm(9, sc.nx, sc.ny, sc.dx)
+        sc.flags |= S.E4_STORM
+    prec = M.F32 if prec_name == "f32" else M.F64
+    _say(f"per-cell boundaries {prec_name} storm={storm} {sc.nx}x{sc.ny}")
+    a = M.Catchment(sc, precision=prec)
+    a.solve()
+    b = M.Catchment(sc, precision=prec)
+    b.params.discharge, b.params.head = 2, 2
+    band = np.zeros((sc.ny, sc.nx), bool)
+    band[:, :sc.head_cols] = True
+    sd = float(sc.step_duration)
+    b.dt = float(sc.initial_dt)
+    for i in range(b.nsteps):
+        t = 0.0 + i * sd
+        q0, q1, hb0, hb1 = b.forcing_at(t)
+        b.upload("qbdy0", np.full((sc.ny, sc.nx), q0))
+        b.upload("qbdy1", np.full((sc.ny, sc.nx), q1))
+        b.upload("hbdy0", np.where(band, hb0, np.nan))
+        b.upload("hbdy1", np.where(band, hb1, np.nan))
+        if i == 0:
+            b._check(M.lib().mfsv_set_params(b._h, C.byref(b.params)))
+        b._check(M.lib().mfsv_start_new_step(b._h, t, q0, q1, hb0, hb1))
+        b.step()
+    assert (a.n_acc, a.n_rej) == (b.n_acc, b.n_rej)
+    for x, y in zip(a.state(), b.state()):
+        np.testing.assert_array_equal(x, y)
+    _say(f"  ok: {a.n_acc} accepted, {a.n_rej} rejected, wet cells {int(np.sum(a.state()[0] > 1e-6))}")
+    a.close()
+    b.close()
+
+
+def _f32(a):
+    return np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("nx,ny,flags,seed", [
+    (64, 48, 0, 21),
+    (61, 17, S.E7_Y_ORIENT | S.E8_DRY_VELOCITY | S.E9_VELOCITY_CAP, 22),
+    (130, 33, S.E7_Y_ORIENT | S.E1_RAIN_GUARD, 23),
+])
+def test_dudt_f32_matches_oracle(gpu_lib, nx, ny, flags, seed):
+    """The FP32 precision form's dU/dt against the FP64 oracle on the same
+    (FP32-representable) inputs, power-of-two spacing: FP32 rounding only,
+    |k32 - k64| <= 1e-5 (1 + |k64|) everywhere and every value finite."""
+    from mfcm_amd.mfsv import Catchment, F32
+    zb, h, u, v = (_f32(a) for a in _random_state(nx, ny, seed))
+    sc = _scenario_from_state(nx, ny, 2.0, 4.0, zb, h, u, v, flags=flags, rain=2e-5,
+                              head_cols=1 if seed % 2 else 0)
+    _say(f"dudt f32 nx={nx} ny={ny} flags={flags} seed={seed}")
+    sim = _oracle(sc)
+    sim.start_new_step(30.0)
+    dt, t_sub = 0.375, 4.5
+    sim.update_dudt(0, t_sub, dt)
+    cat = Catchment(sc, precision=F32)
+    cat.start_new_step(30.0)
+    kh, ku, kv = cat.dudt(t_sub, dt)
+    for name, g in (("dhdt", kh), ("dudt", ku), ("dvdt", kv)):
+        o = sim.array(name).reshape(ny, nx)
+        assert np.all(np.isfinite(g)), name
+        err = np.abs(g - o) / (1.0 + np.abs(o))
+        _say(f"  {name}: max rel err {err.max():.3g}")
+        assert err.max() <= 1e-5, (name, err.max(), np.unravel_index(err.argmax(), err.shape))
+    cat.close()
+
+
+@pytest.mark.parametrize("e7", [False, True])
+def test_run_f32_close_to_oracle(gpu_lib, e7):
+    """A dam break around a bump (flow along both axes) for 30 s in the FP32
+    form against the FP64 oracle: the depth stays within 5e-4 m everywhere
+    (FP32 rounding over ~100 timesteps; an error in the y treatment moves
+    the front by whole cells), and the volume within 1e-6."""
+    from mfcm_amd.mfsv import Catchment, F32
+    sc = S.dam_break_bump(64, 48, dx=4.0, dy=4.0)
+    sc.zb = _f32(sc.zb)
+    sc.t_end = 30.0
+    if e7:
+        sc.flags |= S.E7_Y_ORIENT
+    _say(f"run f32 dam break e7={e7}")
+    sim = _oracle(sc)
+    sim.solve()
+    cat = Catchment(sc, precision=F32)
+    cat.solve()
+    hg = cat.state()[0]
+    ho = sim.get_state(0)[0]
+    err = np.abs(hg - ho).max()
+    _say(f"  max |dh| {err:.3g} m; timesteps {cat.n_acc}+{cat.n_rej} vs {sim.stats()}")
+    assert err <= 5e-4, err
+    vg, vo = cat.volume(), sim.volume(0)
+    assert abs(vg - vo) <= 1e-6 * vo, (vg, vo)
+    cat.close()
