@@ -322,6 +322,17 @@ __device__ __forceinline__ T minmod_q(T ql, T qr, T qc, bool hasl, bool hasr) {
   const T in = minmod_c(ql, qr, qc);
   return hasl ? (hasr ? in : ql) : (hasr ? qr : T(0));
 }
+// Half of minmod_c / minmod_q, exactly: med3(med3(ql, qr, 0), qc/4, 0)
+// selects the same operand at half scale.
+template <typename T>
+__device__ __forceinline__ T minmod_ch(T ql, T qr, T qc) {
+  return minmod2(minmod2(ql, qr), T(0.25) * qc);
+}
+template <typename T>
+__device__ __forceinline__ T minmod_qh(T ql, T qr, T qc, bool hasl, bool hasr) {
+  const T in = minmod_ch(ql, qr, qc);
+  return hasl ? (hasr ? in : T(0.5) * ql) : (hasr ? T(0.5) * qr : T(0));
+}
 
 // The Rusanov speed |un + sign(un) c| of FluxKernel.cpp:225-226 with c >= 0:
 // |un| + c when un != 0 (the sum of two same-signed values rounds
@@ -363,7 +374,10 @@ __device__ __forceinline__ Side<T> wall_ghost(const Side<T>& c) {
 // faces pass a wall_ghost): branches 1, 3 and 4 as one Rusanov form over the
 // wet sides (a dry side contributes zero, and a side that is wet alone takes
 // +g h on the plus side as branch 4 does), branch 2 selected after.
-template <typename T, bool DEPTHREL, bool QUICK>
+// X2: every component doubled (the halvings of the Rusanov average and of
+// the wall force dropped, 2 g for g, and rd passed doubled), exactly twice
+// the X2 = false result: the caller halves once per cell instead.
+template <typename T, bool DEPTHREL, bool QUICK, bool X2 = false>
 __device__ __forceinline__ Flux<T> face_fast(const Side<T>& L, const Side<T>& R, T hd, T d, T rd,
                                              float* branch) {
   const T zm = mad_t(hd, L.sz, L.z);
@@ -401,18 +415,30 @@ __device__ __forceinline__ Flux<T> face_fast(const Side<T>& L, const Side<T>& R,
     const T nfm = mad_t(unme, T(0.5) * unme, gm);
     const T nfp = mad_t(unpe, T(0.5) * unpe, gp);
     const T a = fmax_t(rus_speed(unpe, cp), rus_speed(unme, cm));
-    const T ha = T(0.5) * a;
-    F.h = mad_t(-ha, hpe - hme, T(0.5) * mad_t(hpe, unpe, hme * unme));
-    F.n = mad_t(-ha, unpe - unme, T(0.5) * (nfp + nfm));
-    F.t = mad_t(-ha, utpe - utme, T(0.5) * mad_t(utpe, unpe, utme * unme));
-    F.z = (wm && wp) ? (zm - zp) * G : dv<QUICK>(wm ? hp : hm, d, rd);
+    if constexpr (X2) {
+      F.h = mad_t(-a, hpe - hme, mad_t(hpe, unpe, hme * unme));
+      F.n = mad_t(-a, unpe - unme, nfp + nfm);
+      F.t = mad_t(-a, utpe - utme, mad_t(utpe, unpe, utme * unme));
+      F.z = (wm && wp) ? (zm - zp) * (T(2) * G) : dv<QUICK>(wm ? hp : hm, d, rd);
+    } else {
+      const T ha = T(0.5) * a;
+      F.h = mad_t(-ha, hpe - hme, T(0.5) * mad_t(hpe, unpe, hme * unme));
+      F.n = mad_t(-ha, unpe - unme, T(0.5) * (nfp + nfm));
+      F.t = mad_t(-ha, utpe - utme, T(0.5) * mad_t(utpe, unpe, utme * unme));
+      F.z = (wm && wp) ? (zm - zp) * G : dv<QUICK>(wm ? hp : hm, d, rd);
+    }
     *branch = (wm && wp) ? 1.0f : (wm ? 3.0f : 4.0f);
   } else {
     const bool r = zp > zm;
     F.h = T(0);
     F.t = T(0);
-    F.n = r ? T(0.5) * ghm : T(-0.5) * ghp;
-    F.z = r ? -hm * T(0.5) * G : hp * T(0.5) * G;
+    if constexpr (X2) {
+      F.n = r ? ghm : -ghp;
+      F.z = r ? -hm * G : hp * G;
+    } else {
+      F.n = r ? T(0.5) * ghm : T(-0.5) * ghp;
+      F.z = r ? -hm * T(0.5) * G : hp * T(0.5) * G;
+    }
     *branch = r ? 2.25f : 2.75f;
   }
   return F;

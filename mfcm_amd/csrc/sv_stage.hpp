@@ -287,8 +287,27 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   // the y half spacing and the y pressure and bed factors instead (exact: a
   // product changes sign with either factor)
   constexpr bool kFoldY = sizeof(T) == 4 && QUICK && !NANBED && !EXT && !KOUT;
-  const T hdy_s = (kFoldY && !e7) ? -hdy : hdy;
-  const T gy_s = (kFoldY && !e7) ? -gy : gy;
+  // ... and, under the same conditions, the slopes are carried halved and
+  // the face fluxes doubled (exact power-of-two scalings: the results are
+  // those of the unscaled form): the half spacing of the reconstructions
+  // becomes 1, the pressure and bed factors double, the flux divergence
+  // takes half the reciprocal spacing; this drops a doubling per slope and
+  // the halvings of the Rusanov average per face
+  constexpr bool kScaled = kFoldY;
+  const T hdx_s = kScaled ? T(1) : hdx;
+  const T hdy_s = (kFoldY && !e7) ? -(kScaled ? T(1) : hdy) : (kScaled ? T(1) : hdy);
+  const T gx_s = kScaled ? T(2) * gx : gx;
+  const T gy_s = ((kFoldY && !e7) ? -gy : gy) * (kScaled ? T(2) : T(1));
+  const T rdx_f = kScaled ? T(2) * rdx : rdx, rdy_f = kScaled ? T(2) * rdy : rdy;
+  const T rdx_c = kScaled ? T(0.5) * rdx : rdx, rdy_c = kScaled ? T(0.5) * rdy : rdy;
+  auto mm_c = [&](T ql, T qr, T qc) -> T {
+    if constexpr (kScaled) return minmod_ch(ql, qr, qc);
+    else return minmod_c(ql, qr, qc);
+  };
+  auto mm_q = [&](T ql, T qr, T qc, bool hl, bool hr) -> T {
+    if constexpr (kScaled) return minmod_qh(ql, qr, qc, hl, hr);
+    else return minmod_q(ql, qr, qc, hl, hr);
+  };
   const T G = T(kG);
 
   // RK coefficients of this stage: (T)(a * dt) as State::operator*(ValueType).
@@ -456,15 +475,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       // wave-uniform: no lane of the strip lies on an x edge (the FP64 kernel
       // keeps the selects: the branch costs it more than it saves)
       if (sizeof(T) == 4 && xint) {
-        shx = minmod_c(qwh, qeh, qx(E.h - W.h));
-        sux = minmod_c(qwu, qeu, qx(E.u - W.u));
-        svx = minmod_c(qwv, qev, qx(E.v - W.v));
-        szx = minmod_c(qwz, qez, qx(E.z - W.z));
+        shx = mm_c(qwh, qeh, qx(E.h - W.h));
+        sux = mm_c(qwu, qeu, qx(E.u - W.u));
+        svx = mm_c(qwv, qev, qx(E.v - W.v));
+        szx = mm_c(qwz, qez, qx(E.z - W.z));
       } else {
-        shx = minmod_q(qwh, qeh, qx(E.h - W.h), hasW, hasE);
-        sux = minmod_q(qwu, qeu, qx(E.u - W.u), hasW, hasE);
-        svx = minmod_q(qwv, qev, qx(E.v - W.v), hasW, hasE);
-        szx = minmod_q(qwz, qez, qx(E.z - W.z), hasW, hasE);
+        shx = mm_q(qwh, qeh, qx(E.h - W.h), hasW, hasE);
+        sux = mm_q(qwu, qeu, qx(E.u - W.u), hasW, hasE);
+        svx = mm_q(qwv, qev, qx(E.v - W.v), hasW, hasE);
+        szx = mm_q(qwz, qez, qx(E.z - W.z), hasW, hasE);
       }
       qnh = qy(R2.h - R1.h); qnu = qy(R2.u - R1.u); qnv = qy(R2.v - R1.v); qnz = qy(R2.z - R1.z);
       // south -> north slope; the reference's north -> south slope is its
@@ -477,15 +496,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       }
       T yh, yu, yv, yz;
       if (sizeof(T) == 4 && hasS && hasN) {  // wave-uniform: an interior row
-        yh = minmod_c(qsh, qnh, qy(R2.h - R0.h));
-        yu = minmod_c(qsu, qnu, qy(R2.u - R0.u));
-        yv = minmod_c(qsv, qnv, qy(R2.v - R0.v));
-        yz = minmod_c(qsz, qnz, qy(R2.z - R0.z));
+        yh = mm_c(qsh, qnh, qy(R2.h - R0.h));
+        yu = mm_c(qsu, qnu, qy(R2.u - R0.u));
+        yv = mm_c(qsv, qnv, qy(R2.v - R0.v));
+        yz = mm_c(qsz, qnz, qy(R2.z - R0.z));
       } else {
-        yh = minmod_q(qsh, qnh, qy(R2.h - R0.h), hasS, hasN);
-        yu = minmod_q(qsu, qnu, qy(R2.u - R0.u), hasS, hasN);
-        yv = minmod_q(qsv, qnv, qy(R2.v - R0.v), hasS, hasN);
-        yz = minmod_q(qsz, qnz, qy(R2.z - R0.z), hasS, hasN);
+        yh = mm_q(qsh, qnh, qy(R2.h - R0.h), hasS, hasN);
+        yu = mm_q(qsu, qnu, qy(R2.u - R0.u), hasS, hasN);
+        yv = mm_q(qsv, qnv, qy(R2.v - R0.v), hasS, hasN);
+        yz = mm_q(qsz, qnz, qy(R2.z - R0.z), hasS, hasN);
       }
       if constexpr (kFoldY) {
         shy = yh; suy = yu; svy = yv; szy = yz;
@@ -525,7 +544,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           L = sel(x == 0, e2 ? outflow_ghost(R) : wall_ghost(R), L);
           Rf = sel(x == nx, wall_ghost(L), R);
         }
-        F = face_fast<T, kRel, QUICK>(L, Rf, hdx, dx, rdx, &br);
+        F = face_fast<T, kRel, QUICK, kScaled>(L, Rf, hdx_s, dx, rdx_f, &br);
       } else {
         if (x == 0 && e2) {
           F = face_flux<T, kRel>(outflow_ghost(R), R, 0, hdx, dx, rdx, &br);
@@ -555,16 +574,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       Flux<T> F;
       if (c == ny) {  // north edge of the top row
         const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
-        if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(L, wall_ghost(L), hdy_s, dy, rdy, &br);
+        if constexpr (!NANBED) F = face_fast<T, kRel, QUICK, kScaled>(L, wall_ghost(L), hdy_s, dy, rdy_f, &br);
         else F = face_flux<T, kRel>(L, L, 1, hdy, dy, rdy, &br);
       } else {
         const Side<T> R = {R1.h, R1.v, R1.u, R1.z, shy, svy, suy, szy};
         if (c == 0) {
-          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(wall_ghost(R), R, hdy_s, dy, rdy, &br);
+          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK, kScaled>(wall_ghost(R), R, hdy_s, dy, rdy_f, &br);
           else F = face_flux<T, kRel>(R, R, -1, hdy, dy, rdy, &br);
         } else {
           const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
-          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK>(L, R, hdy_s, dy, rdy, &br);
+          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK, kScaled>(L, R, hdy_s, dy, rdy_f, &br);
           else F = face_flux<T, kRel>(L, R, 0, hdy, dy, rdy, &br);
         }
       }
@@ -604,26 +623,28 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       const uint32_t wo = lane_off + uint32_t(r) * row_bytes;
       const T h = R0.h, u = R0.u, v = R0.v;
       // TemporalDerivativeKernel.cpp:57-90
-      T dhdt = dv2<QUICK>(Xw0.h - Xe0.h, dx, rdx, Fs0.h - Fs1.h, dy, rdy);
-      T dudt = dv2<QUICK>(Xw0.u - Xe0.u, dx, rdx, Fs0.u - Fs1.u, dy, rdy);
-      T dvdt = dv2<QUICK>(Xw0.v - Xe0.v, dx, rdx, Fs0.v - Fs1.v, dy, rdy);
-      dudt = mad_t(p_hx, gx, dudt);
+      T dhdt = dv2<QUICK>(Xw0.h - Xe0.h, dx, rdx_c, Fs0.h - Fs1.h, dy, rdy_c);
+      T dudt = dv2<QUICK>(Xw0.u - Xe0.u, dx, rdx_c, Fs0.u - Fs1.u, dy, rdy_c);
+      T dvdt = dv2<QUICK>(Xw0.v - Xe0.v, dx, rdx_c, Fs0.v - Fs1.v, dy, rdy_c);
+      dudt = mad_t(p_hx, gx_s, dudt);
       dvdt = mad_t(p_hy, gy_s, dvdt);
       // bed slope limited to |dz/dx| <= h/dx
       T dudt_bed, dvdt_bed;
       if constexpr (QUICK) {
         T zx, zy;
         if constexpr (sizeof(T) == 4) {
-          const T hp = fmax_t(h, T(0));
-          zx = bed_limit(p_zx, h, hp);
-          zy = bed_limit(p_zy, h, hp);
+          // half slopes are limited to half the depth
+          const T hl = kScaled ? T(0.5) * h : h;
+          const T hp = fmax_t(hl, T(0));
+          zx = bed_limit(p_zx, hl, hp);
+          zy = bed_limit(p_zy, hl, hp);
         } else {
           // sycl::sign, not copysign: a negative depth passes the test with a
           // zero slope, whose sign is zero
           zx = (fabs_t(p_zx) > h) ? sy_sign(p_zx) * h : p_zx;
           zy = (fabs_t(p_zy) > h) ? sy_sign(p_zy) * h : p_zy;
         }
-        dudt_bed = gx * zx;
+        dudt_bed = gx_s * zx;
         dvdt_bed = gy_s * zy;
       } else {
         T dzdx = p_zx;
@@ -636,8 +657,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         dvdt_bed = T(-kG) * dzdy;
       }
       if constexpr (QUICK) {
-        dudt = mad_t(Xw0.z - Xe0.z, rdx, dudt);
-        dvdt = mad_t(Fs0.z - Fs1.z, rdy, dvdt);
+        dudt = mad_t(Xw0.z - Xe0.z, rdx_c, dudt);
+        dvdt = mad_t(Fs0.z - Fs1.z, rdy_c, dvdt);
       } else {
         dudt += div_c(Xw0.z - Xe0.z, dx, rdx);
         dvdt += div_c(Fs0.z - Fs1.z, dy, rdy);
