@@ -398,43 +398,47 @@ __device__ __forceinline__ Flux<T> face_fast(const Side<T>& L, const Side<T>& R,
     wm = zm + hm > zf;
     wp = zp + hp > zf;
   }
-  hm = fmax_t(hm, T(0));
-  hp = fmax_t(hp, T(0));
   const T G = T(kG);
-  const T ghm = G * hm, ghp = G * hp;
   Flux<T> F;
   if (wm || wp) {
+    // a dry side contributes zero: depth, velocities and with them g h and
+    // the celerity are selected to zero (a wet side's depth is > 0, so the
+    // reference's clamp to >= 0 changes nothing there)
     const T hme = wm ? hm : T(0), unme = wm ? unm : T(0), utme = wm ? utm : T(0);
     const T hpe = wp ? hp : T(0), unpe = wp ? unp : T(0), utpe = wp ? utp : T(0);
-    const T cm = wm ? fsqrt_t(ghm) : T(0);
-    const T cp = wp ? fsqrt_t(ghp) : T(0);
-    const T gm = wm ? ghm : T(0);
-    const T gp = wp ? (wm ? -ghp : ghp) : T(0);
+    const T gm = G * hme, gpe = G * hpe;
+    const T cm = fsqrt_t(gm), cp = fsqrt_t(gpe);
+    const T gp = wm ? -gpe : gpe;
     // mad_t: the FP64 form rounds the product (the reference's operation
     // order), the FP32 form fuses it
     const T nfm = mad_t(unme, T(0.5) * unme, gm);
     const T nfp = mad_t(unpe, T(0.5) * unpe, gp);
     const T a = fmax_t(rus_speed(unpe, cp), rus_speed(unme, cm));
+    // one side dry: the wall force of the dry side's depth, clamped to >= 0
+    const T hdry = fmax_t(wm ? hp : hm, T(0));
     if constexpr (X2) {
       F.h = mad_t(-a, hpe - hme, mad_t(hpe, unpe, hme * unme));
       F.n = mad_t(-a, unpe - unme, nfp + nfm);
       F.t = mad_t(-a, utpe - utme, mad_t(utpe, unpe, utme * unme));
-      F.z = (wm && wp) ? (zm - zp) * (T(2) * G) : dv<QUICK>(wm ? hp : hm, d, rd);
+      F.z = (wm && wp) ? (zm - zp) * (T(2) * G) : dv<QUICK>(hdry, d, rd);
     } else {
       const T ha = T(0.5) * a;
       F.h = mad_t(-ha, hpe - hme, T(0.5) * mad_t(hpe, unpe, hme * unme));
       F.n = mad_t(-ha, unpe - unme, T(0.5) * (nfp + nfm));
       F.t = mad_t(-ha, utpe - utme, T(0.5) * mad_t(utpe, unpe, utme * unme));
-      F.z = (wm && wp) ? (zm - zp) * G : dv<QUICK>(wm ? hp : hm, d, rd);
+      F.z = (wm && wp) ? (zm - zp) * G : dv<QUICK>(hdry, d, rd);
     }
     *branch = (wm && wp) ? 1.0f : (wm ? 3.0f : 4.0f);
   } else {
+    hm = fmax_t(hm, T(0));                   // FluxKernel.cpp:199-204
+    hp = fmax_t(hp, T(0));
+    const T ghm = G * hm, ghp = G * hp;
     const bool r = zp > zm;
     F.h = T(0);
     F.t = T(0);
     if constexpr (X2) {
       F.n = r ? ghm : -ghp;
-      F.z = r ? -hm * G : hp * G;
+      F.z = r ? -ghm : ghp;                  // -hm G == -(G hm): the same product
     } else {
       F.n = r ? T(0.5) * ghm : T(-0.5) * ghp;
       F.z = r ? -hm * T(0.5) * G : hp * T(0.5) * G;
