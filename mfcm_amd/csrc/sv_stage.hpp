@@ -673,7 +673,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           if (A.mfull) { ndp = ld(A.ndp_f, wo); dsh = ld(A.dsh_f, wo); ddp = ld(A.ddp_f, wo); }
         }
         T n = nsh;  // mix(a, a, t) == a
-        if (nsh != ndp) {
+        // one roughness field without per-cell deep n: n_deep == n_shallow
+        // in every cell, the mix is the identity (a wave-uniform skip)
+        bool may_mix = !has_n;
+        if constexpr (EXT) may_mix = may_mix || A.mfull;
+        if (may_mix && nsh != ndp) {
           T t = vdiv(h - dsh, ddp - dsh);
           t = clamp_t(t, T(0), T(1));
           n = mad_t(ndp - nsh, t * t * (T(3) - T(2) * t), nsh);
@@ -683,12 +687,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         const T Sf = n * n * fsqrt_t(mad_t(u, u, v * v)) * pow43(inv_h);
         T dum = T(-kG) * Sf * u;
         T dvm = T(-kG) * Sf * v;
-        // overshoot cap with the full timestep
+        // overshoot cap with the full timestep (ManningRoughnessSourceTerm.cpp:74-81):
+        // pu = ((-g Sf) u) dt with Sf >= 0 (a product of n^2, a square root
+        // and a power; NaN fails every test), so where |pu| > |u| both are
+        // non-zero and pu's sign is the opposite of u's: the reference's sign
+        // test always passes there and is not repeated
         const T pu = dum * dtT, pv = dvm * dtT;
-        // pu is a multiple of u: where |pu| > |u| both are non-zero and not
-        // NaN, and their signs differ exactly when their sign bits do
-        if (fabs_t(pu) > fabs_t(u) && signs_differ(pu, u)) dum = div_dt(-u, dtT, rdt);
-        if (fabs_t(pv) > fabs_t(v) && signs_differ(pv, v)) dvm = div_dt(-v, dtT, rdt);
+        if (fabs_t(pu) > fabs_t(u)) dum = div_dt(-u, dtT, rdt);
+        if (fabs_t(pv) > fabs_t(v)) dvm = div_dt(-v, dtT, rdt);
         dudt = wet ? dudt + dum : dudt;
         dvdt = wet ? dvdt + dvm : dvdt;
         if constexpr (KOUT) {
