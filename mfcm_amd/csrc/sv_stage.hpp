@@ -472,18 +472,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     if constexpr (!NANBED) {
       const T qwh = qx(R1.h - W.h), qwu = qx(R1.u - W.u), qwv = qx(R1.v - W.v), qwz = qx(R1.z - W.z);
       const T qeh = from_e(qwh), qeu = from_e(qwu), qev = from_e(qwv), qez = from_e(qwz);
+      // central differences, once for both forms below (the east shift then
+      // feeds a single subtraction)
+      const T qch = qx(E.h - W.h), qcu = qx(E.u - W.u), qcv = qx(E.v - W.v), qcz = qx(E.z - W.z);
       // wave-uniform: no lane of the strip lies on an x edge (the FP64 kernel
       // keeps the selects: the branch costs it more than it saves)
       if (sizeof(T) == 4 && xint) {
-        shx = mm_c(qwh, qeh, qx(E.h - W.h));
-        sux = mm_c(qwu, qeu, qx(E.u - W.u));
-        svx = mm_c(qwv, qev, qx(E.v - W.v));
-        szx = mm_c(qwz, qez, qx(E.z - W.z));
+        shx = mm_c(qwh, qeh, qch);
+        sux = mm_c(qwu, qeu, qcu);
+        svx = mm_c(qwv, qev, qcv);
+        szx = mm_c(qwz, qez, qcz);
       } else {
-        shx = mm_q(qwh, qeh, qx(E.h - W.h), hasW, hasE);
-        sux = mm_q(qwu, qeu, qx(E.u - W.u), hasW, hasE);
-        svx = mm_q(qwv, qev, qx(E.v - W.v), hasW, hasE);
-        szx = mm_q(qwz, qez, qx(E.z - W.z), hasW, hasE);
+        shx = mm_q(qwh, qeh, qch, hasW, hasE);
+        sux = mm_q(qwu, qeu, qcu, hasW, hasE);
+        svx = mm_q(qwv, qev, qcv, hasW, hasE);
+        szx = mm_q(qwz, qez, qcz, hasW, hasE);
       }
       qnh = qy(R2.h - R1.h); qnu = qy(R2.u - R1.u); qnv = qy(R2.v - R1.v); qnz = qy(R2.z - R1.z);
       // south -> north slope; the reference's north -> south slope is its
@@ -916,8 +919,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           // State.cpp:211-222 on the new state; max_i dt*q_i == dt*max_i q_i
           const T cc = fsqrt_t(G * fmax_t(oh, T(0)));
           const T q = dv2<QUICK>(fabs_t(ou) + cc, dx, rdx, fabs_t(ov) + cc, dy, rdy);
-          qmax = (q > qmax) ? q : qmax;
-          vol += double(oh) + double(ol);
+          qmax = fmax_t(qmax, q);  // q >= 0; a NaN q is skipped, as by q > qmax
+          if constexpr (sizeof(T) == 4) vol += double(oh) + double(ol);
+          else vol += oh;  // ol == 0: the sum is the same
         }
       }
     }
