@@ -112,6 +112,13 @@ __device__ __forceinline__ float clamp_t(float x, float lo, float hi) {
 __device__ __forceinline__ double clamp_t(double x, double lo, double hi) {
   return __builtin_fmin(__builtin_fmax(x, lo), hi);
 }
+// max(x, 0): one median instruction in FP32 (fmax of a value that the
+// compiler cannot prove canonical costs a second instruction that quiets a
+// signalling NaN first); a NaN gives 0 in both forms.
+__device__ __forceinline__ float max0(float x) {
+  return __builtin_amdgcn_fmed3f(x, 0.0f, __builtin_huge_valf());
+}
+__device__ __forceinline__ double max0(double x) { return __builtin_fmax(x, 0.0); }
 // Sign bits differ.
 __device__ __forceinline__ bool signs_differ(float a, float b) {
   return (__float_as_int(a) ^ __float_as_int(b)) < 0;
@@ -415,7 +422,7 @@ __device__ __forceinline__ Flux<T> face_fast(const Side<T>& L, const Side<T>& R,
     const T nfp = mad_t(unpe, T(0.5) * unpe, gp);
     const T a = fmax_t(rus_speed(unpe, cp), rus_speed(unme, cm));
     // one side dry: the wall force of the dry side's depth, clamped to >= 0
-    const T hdry = fmax_t(wm ? hp : hm, T(0));
+    const T hdry = max0(wm ? hp : hm);
     if constexpr (X2) {
       F.h = mad_t(-a, hpe - hme, mad_t(hpe, unpe, hme * unme));
       F.n = mad_t(-a, unpe - unme, nfp + nfm);
@@ -430,8 +437,8 @@ __device__ __forceinline__ Flux<T> face_fast(const Side<T>& L, const Side<T>& R,
     }
     *branch = (wm && wp) ? 1.0f : (wm ? 3.0f : 4.0f);
   } else {
-    hm = fmax_t(hm, T(0));                   // FluxKernel.cpp:199-204
-    hp = fmax_t(hp, T(0));
+    hm = max0(hm);                           // FluxKernel.cpp:199-204
+    hp = max0(hp);
     const T ghm = G * hm, ghp = G * hp;
     const bool r = zp > zm;
     F.h = T(0);

@@ -917,7 +917,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         }
         if (A.last) {
           // State.cpp:211-222 on the new state; max_i dt*q_i == dt*max_i q_i
-          const T cc = fsqrt_t(G * fmax_t(oh, T(0)));
+          const T cc = fsqrt_t(G * max0(oh));
           const T q = dv2<QUICK>(fabs_t(ou) + cc, dx, rdx, fabs_t(ov) + cc, dy, rdy);
           qmax = fmax_t(qmax, q);  // q >= 0; a NaN q is skipped, as by q > qmax
           if constexpr (sizeof(T) == 4) vol += double(oh) + double(ol);
