@@ -224,12 +224,14 @@ __device__ __forceinline__ Side<T> outflow_ghost(const Side<T>& c) {
 }
 
 // NACC: accumulator rows this stage writes (0 with KOUT: write k instead).
-// QUICK: power-of-two spacing (always in the FP32 form).  Slopes are then
-// carried as per-cell differences m = d * slope: for d = 2^k the Minmod3 of
-// the quotients is exactly the Minmod3 of the differences scaled by 1/d, the
-// face projection (d/2) * slope is exactly m/2, and -g * slope is exactly
-// m * (-g/d), so every value is the reference's bit for bit.  Otherwise
-// (and with coded-out cells, NANBED) slopes are exact quotients.
+// QUICK: the difference form, for power-of-two spacing in FP64 and for any
+// spacing in the FP32 form without coded-out cells.  Slopes are then carried
+// as per-cell differences m = d * slope: for d = 2^k the Minmod3 of the
+// quotients is exactly the Minmod3 of the differences scaled by 1/d, the face
+// projection (d/2) * slope is exactly m/2, and -g * slope is exactly
+// m * (-g/d), so every value is the reference's bit for bit (other spacings
+// in FP32 multiply by the once-rounded reciprocal).  Otherwise (and with
+// coded-out cells, NANBED) slopes are exact quotients.
 // EXT: the further source terms (energy loss, infiltration, eddy viscosity)
 // and the stage boundary are compiled in (each still switched at run time).
 // BASE: a later stage (s > 0), whose accumulator rows are read back as the
