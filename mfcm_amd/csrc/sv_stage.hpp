@@ -473,7 +473,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     T qnh = T(0), qnu = T(0), qnv = T(0), qnz = T(0);
     if constexpr (!NANBED) {
       const T qwh = qx(R1.h - W.h), qwu = qx(R1.u - W.u), qwv = qx(R1.v - W.v), qwz = qx(R1.z - W.z);
-      const T qeh = from_e(qwh), qeu = from_e(qwu), qev = from_e(qwv), qez = from_e(qwz);
+      // east differences: the west difference one lane east (a single DPP
+      // move in FP32); the FP64 difference form subtracts the east values it
+      // already holds instead (the same operands: bit-identical, and no
+      // NaN-quieting pass over the shifted words before the minimum)
+      T qeh, qeu, qev, qez;
+      if constexpr (sizeof(T) == 8 && QUICK) {
+        qeh = E.h - R1.h; qeu = E.u - R1.u; qev = E.v - R1.v; qez = E.z - R1.z;
+      } else {
+        qeh = from_e(qwh); qeu = from_e(qwu); qev = from_e(qwv); qez = from_e(qwz);
+      }
       // central differences, once for both forms below (the east shift then
       // feeds a single subtraction)
       const T qch = qx(E.h - W.h), qcu = qx(E.u - W.u), qcv = qx(E.v - W.v), qcz = qx(E.z - W.z);
