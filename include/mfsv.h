@@ -157,6 +157,8 @@ typedef struct mfsv_stats {
  * and the SYCL queue selection of sycl.cpp:22-126) ---------------------- */
 const char* mfsv_version(void);
 int mfsv_device_count(int* count);
+/* Each field must stay below 2 GiB (fewer than 2^28 cells in FP64, 2^29 in
+ * FP32): larger grids return MFSV_ERR_ARG. */
 int mfsv_create(const mfsv_grid* grid, int precision, int device, mfsv_ctx** out);
 int mfsv_destroy(mfsv_ctx* ctx);
 const char* mfsv_last_error(const mfsv_ctx* ctx);
