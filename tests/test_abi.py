@@ -42,3 +42,20 @@ def test_argument_errors_without_device():
     assert mfsv.lib().mfsv_create(None, 0, 0, None) == -1
     assert mfsv.lib().mfsv_destroy(None) == -1
     assert mfsv.lib().mfsv_device_count(None) == -1
+
+
+def test_grid_limits_without_device():
+    # sizes and spacings are checked before any device call: each field must
+    # stay below 2 GiB (32-bit byte offsets in the kernels)
+    import ctypes as C
+    h = C.c_void_p()
+    L = mfsv.lib()
+    big64 = mfsv.Grid(16384, 16384, 1.0, 1.0, 0.0, 0.0)      # 2^28 cells x 8 B = 2 GiB
+    assert L.mfsv_create(C.byref(big64), mfsv.F64, 0, C.byref(h)) == -1
+    big32 = mfsv.Grid(32768, 16384, 1.0, 1.0, 0.0, 0.0)      # 2^29 cells x 4 B = 2 GiB
+    assert L.mfsv_create(C.byref(big32), mfsv.F32, 0, C.byref(h)) == -1
+    for bad in (mfsv.Grid(0, 5, 1.0, 1.0, 0.0, 0.0), mfsv.Grid(5, 5, 0.0, 1.0, 0.0, 0.0),
+                mfsv.Grid(5, 5, 1.0, float("nan"), 0.0, 0.0)):
+        assert L.mfsv_create(C.byref(bad), mfsv.F64, 0, C.byref(h)) == -1
+    assert L.mfsv_create(C.byref(mfsv.Grid(5, 5, 1.0, 1.0, 0.0, 0.0)), 7, 0, C.byref(h)) == -1
+    assert not h.value
