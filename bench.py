@@ -330,6 +330,27 @@ def traffic_bytes(name, rows, size):
     return t["bytes_per_launch"]
 
 
+CU_COUNT, ENGINE_CLOCK_HZ = 256, 2.4e9
+
+
+def valu_issue(name, size, stage_us):
+    """The stage kernel against its instruction-issue limit: VALU
+    instructions per launch (rocprofv3 SQ_INSTS_VALU in the committed profile
+    of this workload) at one wave-instruction per clock per CU, over the
+    launch time measured here; None when no profile matches the grid."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)[name]
+        v = t["valu_insts_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+    if t.get("size") != size:
+        return None
+    issue_us = v / (CU_COUNT * ENGINE_CLOCK_HZ) * 1e6
+    return {"valu_insts_per_launch": v, "issue_us": issue_us, "frac": issue_us / stage_us,
+            "clock_ghz": ENGINE_CLOCK_HZ / 1e9, "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
+
+
 def run_to_end(sc, prec, rows=0):
     from mfcm_amd import mfsv
     cat = mfsv.Catchment(sc, precision=prec)
@@ -581,7 +602,8 @@ def main():
                       "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                       "kernel": f"mfsv::k_stage<{'float' if name == 'f32' else 'double'}>",
                       "stage_us_mean": r["stage_ms_mean"] * 1e3,
-                      "bytes_per_cell_update": B[name]}
+                      "bytes_per_cell_update": B[name],
+                      "valu_issue": valu_issue(name, args.size, r["stage_ms_mean"] * 1e3)}
     f32, f64 = res["f32"], res["f64"]
     line = {
         "metric": METRIC,
