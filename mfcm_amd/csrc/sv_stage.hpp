@@ -1002,11 +1002,21 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
   const int ntiles = *ntiles_p;
   const int k0 = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
                                                 (int)(threadIdx.x >> 6));
+  // the member arguments are read through the constant address space: loads
+  // that no store of the kernel can alias, which the compiler may repeat
+  // where it needs a value instead of keeping it (as it does with a kernel's
+  // own arguments) rather than reloading and waiting after the row stores
+#if defined(__HIP_DEVICE_COMPILE__)
+  using ConstArgs = const __attribute__((address_space(4))) StageArgs<T>;
+#else  // the host pass of a kernel body has no GPU address spaces
+  using ConstArgs = const StageArgs<T>;
+#endif
+  ConstArgs* const cargs = (ConstArgs*)margs;
   for (int k = k0; k < ntiles; k += nwaves) {
     const int m = __builtin_amdgcn_readfirstlane(tiles[k].x);
     const int t = __builtin_amdgcn_readfirstlane(tiles[k].y);
+    const StageArgs<T> a = cargs[m];
     if constexpr (GENERAL) {
-      const StageArgs<T>& a = margs[m];
       const bool ext = (a.eloss | a.infil | a.stageb | a.eddy) != 0;
       if (a.nanbed) {
         stage_tile<T, NACC, false, false, true, true, BASE>(a, t, lane);
@@ -1017,7 +1027,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
         continue;
       }
     }
-    stage_tile<T, NACC, false, QUICK, false, false, BASE>(margs[m], t, lane);
+    stage_tile<T, NACC, false, QUICK, false, false, BASE>(a, t, lane);
   }
 }
 
