@@ -398,6 +398,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     T lo;  // FP32 form: low-order depth of the final row's base
     // FP32 form: per-cell boundary endpoints (discharge, head)
     T q0, q1, hb0, hb1;
+    // FP32 form with the further source terms: their per-cell fields
+    T ndp, dsh, ddp, fx, fy, irate, icap, mu, sb0, sb1;
   };
   T* const hl_in = (sizeof(T) == 4 && !KOUT) ? (A.stage == 0 ? (cur ? A.Uhl[1] : A.Uhl[0])
                                                               : (cur ? A.Uhl[0] : A.Uhl[1]))
@@ -414,6 +416,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   const T* const q1p = A.discharge == 2 ? A.q1f : fb;
   const T* const hb0p = A.head == 2 ? A.hb0f : fb;
   const T* const hb1p = A.head == 2 ? A.hb1f : fb;
+  // the further source terms' fields, likewise (FP32 outside derivative
+  // evaluations)
+  constexpr bool kPreExt = sizeof(T) == 4 && EXT && !KOUT;
+  const T* ndpp = fb; const T* dshp = fb; const T* ddpp = fb; const T* fxp = fb; const T* fyp = fb;
+  const T* iratep = fb; const T* icapp = fb; const T* mup = fb; const T* sb0p = fb; const T* sb1p = fb;
+  if constexpr (kPreExt) {
+    if (A.mfull) { ndpp = A.ndp_f; dshp = A.dsh_f; ddpp = A.ddp_f; }
+    if (A.eloss) { fxp = A.fx; fyp = A.fy; }
+    if (A.infil) { iratep = A.irate; icapp = A.icap; }
+    if (A.eddy == 1) mup = A.mu;
+    if (A.stageb) { sb0p = A.sb0; sb1p = A.sb1; }
+  }
   constexpr bool base_ld = BASE;
   auto load_extra = [&](int r) {
     Extra e;
@@ -426,6 +440,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       e.q0 = ld(q0p, o); e.q1 = ld(q1p, o); e.hb0 = ld(hb0p, o); e.hb1 = ld(hb1p, o);
     } else {
       e.q0 = e.q1 = e.hb0 = e.hb1 = T(0);
+    }
+    if constexpr (kPreExt) {
+      e.ndp = ld(ndpp, o); e.dsh = ld(dshp, o); e.ddp = ld(ddpp, o);
+      e.fx = ld(fxp, o); e.fy = ld(fyp, o); e.irate = ld(iratep, o); e.icap = ld(icapp, o);
+      e.mu = ld(mup, o); e.sb0 = ld(sb0p, o); e.sb1 = ld(sb1p, o);
+    } else {
+      e.ndp = e.dsh = e.ddp = e.fx = e.fy = e.irate = e.icap = e.mu = e.sb0 = e.sb1 = T(0);
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j) { e.bh[j] = T(0); e.bu[j] = T(0); e.bv[j] = T(0); }
@@ -684,7 +705,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         T nsh = A.n_sh, ndp = A.n_dp, dsh = A.d_sh, ddp = A.d_dp;
         if (has_n) { nsh = X0.n; ndp = nsh; }
         if constexpr (EXT) {
-          if (A.mfull) { ndp = ld(A.ndp_f, wo); dsh = ld(A.dsh_f, wo); ddp = ld(A.ddp_f, wo); }
+          if (A.mfull) {
+            if constexpr (kPreExt) { ndp = X0.ndp; dsh = X0.dsh; ddp = X0.ddp; }
+            else { ndp = ld(A.ndp_f, wo); dsh = ld(A.dsh_f, wo); ddp = ld(A.ddp_f, wo); }
+          }
         }
         T n = nsh;  // mix(a, a, t) == a
         // one roughness field without per-cell deep n: n_deep == n_shallow
@@ -737,16 +761,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         // EnergyLossSourceTerm.cpp:59-88
         if (A.eloss) {
           const T hyp = fsqrt_t(u * u + v * v);
-          const T due = vcap(T(-0.5) * u * ld(A.fx, wo) * hyp, u);
-          const T dve = vcap(T(-0.5) * v * ld(A.fy, wo) * hyp, v);
+          const T fxv = kPreExt ? X0.fx : ld(A.fx, wo), fyv = kPreExt ? X0.fy : ld(A.fy, wo);
+          const T due = vcap(T(-0.5) * u * fxv * hyp, u);
+          const T dve = vcap(T(-0.5) * v * fyv * hyp, v);
           dudt = wet ? dudt + due : dudt;
           dvdt = wet ? dvdt + dve : dvdt;
         }
         // InfiltrationSourceTerm.cpp:42-64 (the capacity is consumed here)
         if (A.infil) {
-          T di = ld(A.irate, wo) * dtT;
+          T di = (kPreExt ? X0.irate : ld(A.irate, wo)) * dtT;
           di = (di > h) ? h : di;
-          const T capv = ld(A.icap, wo);
+          const T capv = kPreExt ? X0.icap : ld(A.icap, wo);
           di = (di > capv) ? capv : di;
           dhdt -= div_dt(di, dtT, rdt);
           if (own && !(KOUT && A.diag)) st(A.icap, wo, T(capv - di));
@@ -789,7 +814,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
             mu = A.mu_const + A.mu_smag * A.area *
                  fsqrt_t(dudx_ * dudx_ + dvdy_ * dvdy_ + T(0.5) * (t * t));
           } else {
-            mu = ld(A.mu, wo);
+            mu = kPreExt ? X0.mu : ld(A.mu, wo);
           }
           const T due = vcap(-mu * (d2ux + d2uy), u);
           const T dve = vcap(-mu * (d2vx + d2vy), v);
@@ -856,7 +881,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       // StageBoundarySourceTerm.cpp:43-68: the head boundary on level - bed
       if constexpr (EXT) {
         if (A.stageb) {
-          const T h0 = ld(A.sb0, wo) - R0.z, h1 = ld(A.sb1, wo) - R0.z;
+          const T h0 = (kPreExt ? X0.sb0 : ld(A.sb0, wo)) - R0.z;
+          const T h1 = (kPreExt ? X0.sb1 : ld(A.sb1, wo)) - R0.z;
           T target;
           if constexpr (sizeof(T) == 8) {
             const double dhb = div_c(h1 - h0, step_len, A.rstep);
