@@ -19,7 +19,7 @@ import json
 import math
 import os
 
-CANDIDATES = (8, 12, 14, 15, 16, 18, 20, 24, 28, 32, 36, 40, 48)
+CANDIDATES = (4, 6, 8, 12, 14, 15, 16, 18, 20, 24, 28, 32, 36, 40, 48)
 STRIP = 60
 SIMDS = 1024          # 256 CUs x 4 SIMDs on MI355X
 OCCUPANCY = {0: 2, 1: 4}   # waves per SIMD of the stage kernel: F64, F32
