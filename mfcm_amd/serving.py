@@ -112,6 +112,10 @@ class FloodServer:
         return True
 
     def _retire(self, now):
+        # the batch drops finished members from its active set at every chunk
+        # boundary; when none was dropped there is no control block to read
+        if self.batch.size()[1] == len(self.active):
+            return
         for rid in list(self.active):
             req, cat, need, t_adm, ncells = self.active[rid]
             st = cat.stats()
