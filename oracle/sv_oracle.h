@@ -20,7 +20,16 @@
  *   or_face              hand-computed KATs for branches 1, 2.25, 2.75, 3, 4,
  *                        edge faces and coded-out faces; flat-lake invariant
  *   or_td (cell pass)    lake-at-rest => dU/dt == 0 exactly; telescoping sum
- *   or_manning           closed form incl. overshoot cap
+ *   or_manning           closed form incl. overshoot cap; the shallow/deep
+ *                        blend n = mix(n_sh, n_dp, smoothstep) with distinct
+ *                        values below, inside and above [d_sh, d_dp]
+ *   E2 outflow ghost     uniform westward flow passes h u at x = 0 (depth
+ *                        steady there; a closed wall piles up -h u / dx);
+ *                        flow away from the edge sees a ghost of zero normal
+ *                        velocity (Rusanov average h u / 2)
+ *   E4 storm             peak at the drifted centre, exp(-1/2) at one sigma,
+ *                        the separable Gaussian at random points, and its
+ *                        entry into the discharge term at each cell centre
  *   or_discharge/head    closed form incl. the dry guard
  *   energy loss          closed form incl. the overshoot cap and dry cut
  *   infiltration         closed form: rate, depth and capacity limits, the
@@ -29,7 +38,9 @@
  *                        Smagorinsky coefficient of linear fields
  *   stage boundary       closed form: target = level - bed (head invariant)
  *   RK + controller      tableau KATs; Heun-2 on a linear ODE; controller
- *                        table over cn; landing rules
+ *                        table over cn; landing rules: the exact timestep
+ *                        sequence into a step end with and without the
+ *                        40/60 split
  *   volume               exact for constant fields
  * Parity unpinned (no reference test or golden vector exists and the
  * behaviour is implementation-defined in SYCL 1.2.1):

@@ -542,3 +542,180 @@ def test_smagorinsky_coefficient_of_linear_fields(flags):
     assert np.all(mu[1:-1, 1:-1] == want)
     # linear fields: zero second derivatives, no eddy force
     assert np.all(sim.array("d2udx2").reshape(n, n)[1:-1, 1:-1] == 0.0)
+
+
+# ---------------------------------------------------------------- E2 outflow edge
+def _edge_sim(u, flags, nx=6, ny=4, h=0.8, v=0.0, dx=5.0, dy=4.0):
+    sim = O.Sim(nx, ny, dx, dy, "heun2", flags)
+    sim.set_bed(np.zeros((ny, nx)))
+    sim.set_state(np.full((ny, nx), h), np.full((ny, nx), u), np.full((ny, nx), v))
+    sim.set_time(0.0, 60.0, 60.0, 1.0)
+    sim.start_new_step(0.0)
+    sim.update_dudt(0, 0.0, 0.1)
+    return sim
+
+
+def test_e2_uniform_outflow_leaves_through_the_west_edge():
+    """A uniform state flowing west (u < 0) on a flat bed: with E2 the x = 0
+    faces pass the physical flux of that state (h u, u^2/2 in the reference's
+    momentum form, v u), so the edge column keeps its depth; a closed wall
+    (the reference) stops the water there and it piles up at -h u / dx."""
+    h, u, v, dx = 0.8, -0.5, 0.1, 5.0
+    nx, ny = 6, 4
+    nxf = (nx + 1) * ny
+    sim = _edge_sim(u, O.E2_WEST_OUTFLOW, h=h, v=v, dx=dx)
+    west = lambda name: sim.array(name)[:nxf].reshape(ny, nx + 1)[:, 0]
+    assert np.all(west("hflux") == h * u)
+    np.testing.assert_allclose(west("uflux"), 0.5 * u * u, rtol=1e-14, atol=1e-15)
+    assert np.all(west("vflux") == v * u)
+    assert np.all(west("zflux") == 0.0)
+    assert np.all(west("branch") == 1.0)
+    # rows away from the north/south walls (whose v flux drains one wall row
+    # and fills the other)
+    dh = sim.array("dhdt").reshape(ny, nx)
+    assert np.all(dh[1:-1, 0] == 0.0)
+    # the east wall is closed with or without E2 (the flux only leaves west)
+    east = sim.array("hflux")[:nxf].reshape(ny, nx + 1)[:, nx]
+    assert np.all(east == 0.0)
+    wall = _edge_sim(u, 0, h=h, v=v, dx=dx)
+    dhw = wall.array("dhdt").reshape(ny, nx)
+    np.testing.assert_allclose(dhw[1:-1, 0], -h * u / dx, rtol=1e-14)
+
+
+def test_e2_ghost_carries_no_inflow_velocity():
+    """Flow away from the x = 0 edge (u > 0): the E2 ghost takes the cell's
+    depth and tangential velocity but zero normal velocity (min(u, 0)), so the
+    face is the Rusanov average of (h, 0) and (h, u): mass h u / 2, normal
+    momentum (u^2/2)/2 - (|u| + c) u / 2 with c = sqrt(g h).  A ghost that
+    copied u (max instead of min) would pass h u."""
+    h, u, dx = 0.8, 0.5, 5.0
+    nx, ny = 6, 4
+    nxf = (nx + 1) * ny
+    sim = _edge_sim(u, O.E2_WEST_OUTFLOW, h=h, dx=dx)
+    west = lambda name: sim.array(name)[:nxf].reshape(ny, nx + 1)[:, 0]
+    c = math.sqrt(G * h)
+    np.testing.assert_allclose(west("hflux"), 0.5 * h * u, rtol=1e-15)
+    np.testing.assert_allclose(west("uflux"), 0.25 * u * u - 0.5 * (u + c) * u, rtol=1e-13)
+    assert np.all(west("branch") == 1.0)
+
+
+# ---------------------------------------------------------------- E4 storm
+STORM = dict(peak=80.0 / 3.6e6, sigma=400.0, cx0=1000.0, cy0=700.0, vx=3.0, vy=-4.0)
+
+
+def test_e4_storm_rate_closed_form():
+    s = STORM
+    # centre at t = 0, and the centre drifted by (vx t, vy t)
+    assert O.storm_rate(s, s["cx0"], s["cy0"], 0.0) == s["peak"]
+    t = 250.0
+    assert O.storm_rate(s, s["cx0"] + s["vx"] * t, s["cy0"] + s["vy"] * t, t) == s["peak"]
+    # one sigma along each axis, and along both (separable product)
+    want1 = s["peak"] * math.exp(-0.5)
+    assert O.storm_rate(s, s["cx0"] + s["sigma"], s["cy0"], 0.0) == pytest.approx(want1, rel=1e-15)
+    assert O.storm_rate(s, s["cx0"], s["cy0"] - s["sigma"], 0.0) == pytest.approx(want1, rel=1e-15)
+    assert O.storm_rate(s, s["cx0"] + s["sigma"], s["cy0"] + s["sigma"], 0.0) == \
+        pytest.approx(s["peak"] * math.exp(-1.0), rel=1e-15)
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        x, y, tt = rng.uniform(-500, 3000), rng.uniform(-500, 2500), rng.uniform(0, 900)
+        dx, dy = x - (s["cx0"] + s["vx"] * tt), y - (s["cy0"] + s["vy"] * tt)
+        want = s["peak"] * math.exp(-(dx * dx + dy * dy) / (2 * s["sigma"] ** 2))
+        assert O.storm_rate(s, x, y, tt) == pytest.approx(want, rel=1e-13, abs=1e-300)
+    # far from the centre the rate underflows to exactly zero
+    assert O.storm_rate(s, s["cx0"] + 60 * s["sigma"], s["cy0"], 0.0) == 0.0
+
+
+def test_e4_storm_feeds_the_discharge_term():
+    """The storm is sampled at every cell centre at the outer step's two
+    endpoint times and enters the discharge term as a linearly varying rate,
+    averaged over [t_sub, t_sub + dt] (DischargeBoundarySourceTerm.cpp:43-67)."""
+    nx, ny, dx, dy = 5, 3, 200.0, 300.0
+    sim = O.Sim(nx, ny, dx, dy, "heun2", O.E4_STORM | O.E1_RAIN_GUARD)
+    sim.set_bed(np.zeros((ny, nx)))
+    sim.set_state(np.full((ny, nx), 1.0))
+    sim.set_storm(STORM)
+    sim.set_time(0.0, 600.0, 60.0, 1.0)
+    t0, t_sub, dt, L = 120.0, 7.5, 0.5, 60.0
+    sim.start_new_step(t0)
+    sim.update_dudt(0, t_sub, dt)
+    dh = sim.array("dhdt").reshape(ny, nx)
+    s = STORM
+    for y in range(ny):
+        for x in range(nx):
+            cx, cy = (x + 0.5) * dx, (y + 0.5) * dy
+
+            def rate(t):
+                ddx, ddy = cx - (s["cx0"] + s["vx"] * t), cy - (s["cy0"] + s["vy"] * t)
+                return s["peak"] * math.exp(-(ddx * ddx + ddy * ddy) / (2 * s["sigma"] ** 2))
+            q0, q1 = rate(t0), rate(t0 + L)
+            want = q0 + (q1 - q0) / L * (t_sub + 0.5 * dt)
+            assert dh[y, x] == pytest.approx(want, rel=1e-12), (x, y)
+
+
+# ---------------------------------------------------------------- Manning blend
+@pytest.mark.parametrize("h,t", [(0.05, 0.0), (0.1, 0.0), (0.15, 0.25), (0.2, 0.5),
+                                 (0.25, 0.75), (0.3, 1.0), (0.9, 1.0)])
+def test_manning_shallow_deep_blend(h, t):
+    """n = mix(n_sh, n_dp, smoothstep(d_sh, d_dp, h)) with distinct n values
+    (ManningRoughnessSourceTerm.cpp:57-60): n_sh below d_sh, n_dp above d_dp,
+    the Hermite blend t^2 (3 - 2t) between; the friction uses that n."""
+    n_sh, n_dp, d_sh, d_dp = 0.02, 0.08, 0.1, 0.3
+    u, v = 0.6, -0.25
+    sim = O.Sim(1, 1, 10.0, 10.0)
+    sim.set_bed(np.zeros((1, 1)))
+    sim.set_state(np.array([[h]]), np.array([[u]]), np.array([[v]]))
+    sim.set_manning(n_sh, n_dp, d_sh, d_dp)
+    sim.update_dudt(0, 0.0, 0.01)
+    n_want = n_sh + (n_dp - n_sh) * (t * t * (3.0 - 2.0 * t))
+    assert sim.array("mannings_n")[0] == pytest.approx(n_want, rel=1e-14, abs=1e-17)
+    inv_h = h / (h * h + 1e-3)
+    sf = n_want * n_want * math.sqrt(u * u + v * v) * inv_h ** (4.0 / 3.0)
+    assert sim.array("friction_slope")[0] == pytest.approx(sf, rel=1e-13)
+    # no other term acts on a single flat cell away from walls' momentum:
+    # compare with the same cell without friction
+    du, dv, _ = _single(h, u, v, n=n_want, dt=0.01)
+    sim_du, sim_dv = sim.array("dudt")[0], sim.array("dvdt")[0]
+    ref = O.Sim(1, 1, 10.0, 10.0)
+    ref.set_bed(np.zeros((1, 1)))
+    ref.set_state(np.array([[h]]), np.array([[u]]), np.array([[v]]))
+    ref.update_dudt(0, 0.0, 0.01)
+    assert sim_du - ref.array("dudt")[0] == pytest.approx(-G * sf * u, rel=1e-12)
+    assert sim_dv - ref.array("dvdt")[0] == pytest.approx(-G * sf * v, rel=1e-12)
+
+
+# ---------------------------------------------------------------- 40/60 landing
+def _still_lake_dts(period_s, step=60.0, dt0=1.0):
+    """A lake at rest whose control number is dt / period_s (dx = dy = 100 m,
+    h chosen so that 2 sqrt(g h) / dx = 1 / period_s); returns the accepted
+    timesteps of the first outer step."""
+    dx = 100.0
+    c = dx / (2.0 * period_s)
+    h = c * c / G
+    sim = O.Sim(3, 3, dx, dx)
+    sim.set_bed(np.zeros((3, 3)))
+    sim.set_state(np.full((3, 3), h))
+    sim.set_time(0.0, step, step, dt0)
+    sim.enable_trace(100)
+    sim.solve()
+    tr = sim.trace()
+    return tr[tr[:, 3] == 0][:, 1], tr
+
+
+def test_landing_splits_the_last_two_timesteps_40_60():
+    """TemporalScheme.hpp:134-145.  After a 1 s first timestep the controller
+    asks for ~40 s; 1 + 40 <= 60 but 1 + 2*40 > 60, so the next timestep is
+    0.4 of the remaining 59 s and the last one lands on the step end."""
+    dts, tr = _still_lake_dts(40.0)
+    assert len(dts) == 3 and not np.any(tr[:, 3])
+    assert dts[0] == 1.0
+    assert dts[1] == (60.0 - 1.0) * 0.4
+    assert dts[2] == 60.0 - (1.0 + (60.0 - 1.0) * 0.4)
+
+
+def test_landing_goes_straight_to_the_step_end():
+    """When the controller's timestep already overshoots the step end
+    (1 + 80 > 60) the next timestep is the remainder, without the 40/60
+    split."""
+    dts, tr = _still_lake_dts(80.0)
+    assert len(dts) == 2 and not np.any(tr[:, 3])
+    assert dts[0] == 1.0 and dts[1] == 59.0
