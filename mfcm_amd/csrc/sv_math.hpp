@@ -174,113 +174,141 @@ template <typename T> struct Flux {
   T h, n, t, z;
 };
 
-// FluxKernel.cpp:66-295 for one face.  The tangential-slope terms of the
-// reference's projections are multiplied by a zero direction flag and only
-// add a signed zero, so they are left out.  hd = 0.5*d, rd = 1/d.
-// DEPTHREL: wet tests as h_m > max(zb_p - zb_m, 0) (equal to
-// zb_m + h_m > max(zb_m, zb_p) in exact arithmetic) for the FP32 form.
+// A cell as one side of a face sees it in the general form: values and all
+// eight limited slopes (along x and along y).
+template <typename T> struct CellV {
+  T h, u, v, z;
+  T zx, zy, hx, ux, vx, hy, uy, vy;
+};
+
+// A face flux in x/y components: h, u momentum, v momentum, wall force.
+template <typename T> struct FluxXY {
+  T h, u, v, z;
+};
+
+// FluxKernel.cpp:66-295 for one face, term for term, as the catchments with
+// coded-out (NaN-bed) cells need it: the projections keep the tangential
+// slopes multiplied by the zero direction factor and the normal speed keeps
+// the tangential velocity times zero, because next to coded-out cells a
+// slope (or, after a timestep, a velocity) can be NaN and then poisons the
+// reference's face where a finite value would only add a signed zero.
+// dir 0: x-face (d = dx), 1: y-face (d = dy); rd = 1/d.
+// DEPTHREL: wet tests as h_m > max(zb_p - zb_m, 0) (the FP32 form).
 template <typename T, bool DEPTHREL>
-__device__ __forceinline__ Flux<T> face_flux(Side<T> L, Side<T> R, int edge, T hd, T d, T rd,
-                                             float* branch) {
-  Flux<T> F;
-  if (L.z != L.z) {                          // FluxKernel.cpp:85-95
+__device__ __forceinline__ FluxXY<T> face_general(CellV<T> L, CellV<T> R, int edge, int dir, T d,
+                                                  T rd, float* branch) {
+  FluxXY<T> F;
+  const T xdir = T(1) - T(dir), ydir = T(dir);              // :76-77
+  T zb_L = L.z, zb_R = R.z;                                 // :80-81
+  if (zb_L != zb_L) {                                       // :85-95
     L = R;
     if (edge == 1) goto zero;
     edge = -1;
   }
-  if (R.z != R.z) {                          // :96-107
+  if (zb_R != zb_R) {                                       // :96-107
     if (edge == -1) goto zero;
     R = L;
     edge = 1;
   }
   {
-    const T mL = edge < 0 ? T(0) : T(1);     // :111-148
-    const T mR = edge > 0 ? T(0) : T(1);
-    T hL = L.h * mL, hR = R.h * mR;
-    T unL = L.un * mL, unR = R.un * mR;
-    T shL = L.sh * mL, shR = R.sh * mR;
-    T sunL = L.sun * mL, sunR = R.sun * mR;
-    T szL = L.sz * mL, szR = R.sz * mR;
-    T zL = L.z, zR = R.z;
-    if (edge < 0) zL = zR + hR * T(2);       // :152-157
-    if (edge > 0) zR = zL + hL * T(2);
-    // :161-187
-    const T zm = zL + hd * szL;
-    T hm = hL + hd * shL;
-    const T unm = unL + hd * sunL;
-    const T utm = L.ut + hd * L.sut;
-    const T zp = zR - hd * szR;
-    T hp = hR - hd * shR;
-    const T unp = unR - hd * sunR;
-    const T utp = R.ut - hd * R.sut;
-    bool wm, wp;
+    const T mL = edge < 0 ? T(0) : T(1), mR = edge > 0 ? T(0) : T(1);
+    const T mLx = (edge < 0 && dir == 0) ? T(0) : T(1), mRx = (edge > 0 && dir == 0) ? T(0) : T(1);
+    const T mLy = (edge < 0 && dir == 1) ? T(0) : T(1), mRy = (edge > 0 && dir == 1) ? T(0) : T(1);
+    const T h_L = L.h * mL, h_R = R.h * mR;                 // :111-112
+    const T u_L = L.u * mLx, u_R = R.u * mRx;               // :116-117
+    const T v_L = L.v * mLy, v_R = R.v * mRy;               // :121-122
+    const T hx_L = L.hx * mL, hx_R = R.hx * mR;             // :125-128
+    const T hy_L = L.hy * mL, hy_R = R.hy * mR;
+    const T ux_L = L.ux * mLx, ux_R = R.ux * mRx;           // :132-135
+    const T uy_L = L.uy * mLx, uy_R = R.uy * mRx;
+    const T vx_L = L.vx * mLy, vx_R = R.vx * mRy;           // :139-142
+    const T vy_L = L.vy * mLy, vy_R = R.vy * mRy;
+    const T zx_L = L.zx * mL, zx_R = R.zx * mR;             // :145-148
+    const T zy_L = L.zy * mL, zy_R = R.zy * mR;
+    if (edge < 0) zb_L = zb_R + h_R * T(2);                 // :152-157
+    if (edge > 0) zb_R = zb_L + h_L * T(2);
+    // :161-187, evaluated left to right ((0.5 d) s) dir
+    const T hd = T(0.5) * d, mhd = T(-0.5) * d;
+    const T zb_m = zb_L + hd * zx_L * xdir + hd * zy_L * ydir;
+    T h_m = h_L + hd * hx_L * xdir + hd * hy_L * ydir;
+    const T u_m = u_L + hd * ux_L * xdir + hd * uy_L * ydir;
+    const T v_m = v_L + hd * vx_L * xdir + hd * vy_L * ydir;
+    const T zb_p = zb_R + mhd * zx_R * xdir + mhd * zy_R * ydir;
+    T h_p = h_R + mhd * hx_R * xdir + mhd * hy_R * ydir;
+    const T u_p = u_R + mhd * ux_R * xdir + mhd * uy_R * ydir;
+    const T v_p = v_R + mhd * vx_R * xdir + mhd * vy_R * ydir;
+    // :191-195: the wet tests and their dry counterparts, all on the
+    // unclamped projections; a NaN operand fails both, which gives branch 4
+    bool wm, wp, dm, dp;
     if (DEPTHREL) {
-      const T dz = zp - zm;
-      wm = hm > fmax_t(dz, T(0));
-      wp = hp > fmax_t(-dz, T(0));
+      const T dz = zb_p - zb_m;
+      const T lm = fmax_t(dz, T(0)), lp = fmax_t(-dz, T(0));
+      wm = h_m > lm; wp = h_p > lp;
+      dm = h_m <= lm; dp = h_p <= lp;
     } else {
-      const T zf = fmax_t(zm, zp);           // :191-195
-      const T ym = zm + hm;
-      const T yp = zp + hp;
-      wm = ym > zf;
-      wp = yp > zf;
+      const T zb_f = fmax_t(zb_m, zb_p);
+      const T y_m = zb_m + h_m, y_p = zb_p + h_p;
+      wm = y_m > zb_f; wp = y_p > zb_f;
+      dm = y_m <= zb_f; dp = y_p <= zb_f;
     }
-    hm = fmax_t(hm, T(0));                   // :199-204
-    hp = fmax_t(hp, T(0));
+    h_m = fmax_t(h_m, T(0));                                // :199-200
+    h_p = fmax_t(h_p, T(0));
     const T G = T(kG);
-    if (wm && wp) {                          // :208-234
-      const T cm = sqrt_t(G * hm);
-      const T cp = sqrt_t(G * hp);
-      const T hfm = hm * unm, hfp = hp * unp;
-      const T nfm = unm * (T(0.5) * unm) + G * hm;
-      const T nfp = unp * (T(0.5) * unp) - G * hp;
-      const T tfm = utm * unm, tfp = utp * unp;
-      const T a = fmax_t(fabs_t(unp + sy_sign(unp) * cp), fabs_t(unm + sy_sign(unm) * cm));
-      const T ha = T(0.5) * a;
-      F.h = T(0.5) * (hfp + hfm) - ha * (hp - hm);
-      F.n = T(0.5) * (nfp + nfm) - ha * (unp - unm);
-      F.t = T(0.5) * (tfp + tfm) - ha * (utp - utm);
-      F.z = (zm - zp) * G;
+    const T c_m = sqrt_t(G * h_m), c_p = sqrt_t(G * h_p);   // :203-204
+    if (wm && wp) {                                         // :208-234
+      const T spd_m = u_m * xdir + v_m * ydir;
+      const T spd_p = u_p * xdir + v_p * ydir;
+      const T hf_m = h_m * spd_m, hf_p = h_p * spd_p;
+      const T uf_m = u_m * ((T(1) - T(0.5) * xdir) * spd_m) + G * h_m * xdir;
+      const T uf_p = u_p * ((T(1) - T(0.5) * xdir) * spd_p) - G * h_p * xdir;
+      const T vf_m = v_m * ((T(1) - T(0.5) * ydir) * spd_m) + G * h_m * ydir;
+      const T vf_p = v_p * ((T(1) - T(0.5) * ydir) * spd_p) - G * h_p * ydir;
+      const T a = fmax_t(fabs_t(spd_p + sy_sign(spd_p) * c_p), fabs_t(spd_m + sy_sign(spd_m) * c_m));
+      F.h = T(0.5) * (hf_p + hf_m) - T(0.5) * a * (h_p - h_m);
+      F.u = T(0.5) * (uf_p + uf_m) - T(0.5) * a * (u_p - u_m);
+      F.v = T(0.5) * (vf_p + vf_m) - T(0.5) * a * (v_p - v_m);
+      F.z = (zb_m - zb_p) * G;
       *branch = 1.0f;
-    } else if (!wm && !wp) {                 // :235-254
+    } else if (dm && dp) {                                  // :235-254
       F.h = T(0);
-      F.t = T(0);
-      if (zp > zm) {
-        F.n = T(0.5) * (G * hm);
-        F.z = -hm * T(0.5) * G;
+      if (zb_p > zb_m) {
+        F.u = T(0.5) * (G * h_m * xdir);
+        F.v = T(0.5) * (G * h_m * ydir);
+        F.z = -h_m * T(0.5) * G;
         *branch = 2.25f;
       } else {
-        F.n = T(-0.5) * (G * hp);
-        F.z = hp * T(0.5) * G;
+        F.u = T(-0.5) * (G * h_p * xdir);
+        F.v = T(-0.5) * (G * h_p * ydir);
+        F.z = h_p * T(0.5) * G;
         *branch = 2.75f;
       }
-    } else if (wm) {                         // :255-271
-      const T cm = sqrt_t(G * hm);
-      const T hfm = hm * unm;
-      const T nfm = unm * (T(0.5) * unm) + G * hm;
-      const T tfm = utm * unm;
-      const T ha = T(0.5) * fabs_t(unm + sy_sign(unm) * cm);
-      F.h = T(0.5) * hfm - ha * (-hm);
-      F.n = T(0.5) * nfm - ha * (-unm);
-      F.t = T(0.5) * tfm - ha * (-utm);
-      F.z = div_c(hp, d, rd);
+    } else if (wm) {                                        // :255-271
+      const T spd_m = u_m * xdir + v_m * ydir;
+      const T hf_m = h_m * spd_m;
+      const T uf_m = u_m * ((T(1) - T(0.5) * xdir) * spd_m) + G * h_m * xdir;
+      const T vf_m = v_m * ((T(1) - T(0.5) * ydir) * spd_m) + G * h_m * ydir;
+      const T a = fabs_t(spd_m + sy_sign(spd_m) * c_m);
+      F.h = T(0.5) * hf_m - T(0.5) * a * (-h_m);
+      F.u = T(0.5) * uf_m - T(0.5) * a * (-u_m);
+      F.v = T(0.5) * vf_m - T(0.5) * a * (-v_m);
+      F.z = div_c(h_p, d, rd);
       *branch = 3.0f;
-    } else {                                 // :272-289
-      const T cp = sqrt_t(G * hp);
-      const T hfp = hp * unp;
-      const T nfp = unp * (T(0.5) * unp) + G * hp;
-      const T tfp = utp * unp;
-      const T ha = T(0.5) * fabs_t(unp + sy_sign(unp) * cp);
-      F.h = T(0.5) * hfp - ha * hp;
-      F.n = T(0.5) * nfp - ha * unp;
-      F.t = T(0.5) * tfp - ha * utp;
-      F.z = div_c(hm, d, rd);
+    } else {                                                // :272-289
+      const T spd_p = u_p * xdir + v_p * ydir;
+      const T hf_p = h_p * spd_p;
+      const T uf_p = u_p * ((T(1) - T(0.5) * xdir) * spd_p) + G * h_p * xdir;
+      const T vf_p = v_p * ((T(1) - T(0.5) * ydir) * spd_p) + G * h_p * ydir;
+      const T a = fabs_t(spd_p + sy_sign(spd_p) * c_p);
+      F.h = T(0.5) * hf_p - T(0.5) * a * h_p;
+      F.u = T(0.5) * uf_p - T(0.5) * a * u_p;
+      F.v = T(0.5) * vf_p - T(0.5) * a * v_p;
+      F.z = div_c(h_m, d, rd);
       *branch = 4.0f;
     }
     return F;
   }
-zero:                                        // :47-60
-  F.h = T(0); F.n = T(0); F.t = T(0); F.z = T(0);
+zero:                                                       // :47-60
+  F.h = T(0); F.u = T(0); F.v = T(0); F.z = T(0);
   *branch = 0.0f;
   return F;
 }

@@ -580,18 +580,28 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           Rf = sel(x == nx, wall_ghost(L), R);
         }
         F = face_fast<T, kRel, QUICK, kScaled>(L, Rf, hdx_s, dx, rdx_f, &br);
+        Xw1 = {F.h, F.n, F.t, F.z};
       } else {
+        // FluxKernel.cpp:66-295 term for term (coded-out cells)
+        const CellV<T> Rc = {R1.h, R1.u, R1.v, R1.z, szx, szy, shx, sux, svx, shy, suy, svy};
+        CellV<T> Lc = {W.h, W.u, W.v, W.z, from_w(szx), from_w(szy), from_w(shx), from_w(sux),
+                       from_w(svx), from_w(shy), from_w(suy), from_w(svy)};
+        FluxXY<T> G2;
         if (x == 0 && e2) {
-          F = face_flux<T, kRel>(outflow_ghost(R), R, 0, hdx, dx, rdx, &br);
+          // E2 ghost: the cell's depth, bed and tangential velocity, the
+          // normal velocity only outwards, no slopes
+          const CellV<T> Gc = {R1.h, fmin_t(R1.u, T(0)), R1.v, R1.z, T(0), T(0), T(0), T(0),
+                               T(0), T(0), T(0), T(0)};
+          G2 = face_general<T, kRel>(Gc, Rc, 0, 0, dx, rdx, &br);
         } else {
-          Side<T> Rf = R;
+          CellV<T> Rf = Rc;
           int edge = 0;
-          if (x == 0) { L = R; edge = -1; }
-          else if (x == nx) { Rf = L; edge = 1; }
-          F = face_flux<T, kRel>(L, Rf, edge, hdx, dx, rdx, &br);
+          if (x == 0) { Lc = Rc; edge = -1; }
+          else if (x == nx) { Rf = Lc; edge = 1; }
+          G2 = face_general<T, kRel>(Lc, Rf, edge, 0, dx, rdx, &br);
         }
+        Xw1 = {G2.h, G2.u, G2.v, G2.z};
       }
-      Xw1 = {F.h, F.n, F.t, F.z};
       if constexpr (KOUT) {
         if (x >= 0 && x <= nx && lane >= kHalo && lane <= kWave - kHalo) {
           const int64_t fi = int64_t(c) * (nx + 1) + x;
@@ -607,22 +617,34 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     if (c >= y0) {
       float br;
       Flux<T> F;
-      if (c == ny) {  // north edge of the top row
-        const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
-        if constexpr (!NANBED) F = face_fast<T, kRel, QUICK, kScaled>(L, wall_ghost(L), hdy_s, dy, rdy_f, &br);
-        else F = face_flux<T, kRel>(L, L, 1, hdy, dy, rdy, &br);
-      } else {
-        const Side<T> R = {R1.h, R1.v, R1.u, R1.z, shy, svy, suy, szy};
-        if (c == 0) {
-          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK, kScaled>(wall_ghost(R), R, hdy_s, dy, rdy_f, &br);
-          else F = face_flux<T, kRel>(R, R, -1, hdy, dy, rdy, &br);
-        } else {
+      if constexpr (!NANBED) {
+        if (c == ny) {  // north edge of the top row
           const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
-          if constexpr (!NANBED) F = face_fast<T, kRel, QUICK, kScaled>(L, R, hdy_s, dy, rdy_f, &br);
-          else F = face_flux<T, kRel>(L, R, 0, hdy, dy, rdy, &br);
+          F = face_fast<T, kRel, QUICK, kScaled>(L, wall_ghost(L), hdy_s, dy, rdy_f, &br);
+        } else {
+          const Side<T> R = {R1.h, R1.v, R1.u, R1.z, shy, svy, suy, szy};
+          if (c == 0) {
+            F = face_fast<T, kRel, QUICK, kScaled>(wall_ghost(R), R, hdy_s, dy, rdy_f, &br);
+          } else {
+            const Side<T> L = {R0.h, R0.v, R0.u, R0.z, p_hy, p_vy, p_uy, p_zy};
+            F = face_fast<T, kRel, QUICK, kScaled>(L, R, hdy_s, dy, rdy_f, &br);
+          }
         }
+        Fs1 = {F.h, F.t, F.n, F.z};
+      } else {
+        // FluxKernel.cpp:66-295 term for term (coded-out cells); the cell
+        // below is row c-1 with its slopes of the previous iteration
+        const CellV<T> Lc = {R0.h, R0.u, R0.v, R0.z, p_zx, p_zy, p_hx, p_ux, p_vx, p_hy, p_uy, p_vy};
+        FluxXY<T> G2;
+        if (c == ny) {
+          G2 = face_general<T, kRel>(Lc, Lc, 1, 1, dy, rdy, &br);
+        } else {
+          const CellV<T> Rc = {R1.h, R1.u, R1.v, R1.z, szx, szy, shx, sux, svx, shy, suy, svy};
+          if (c == 0) G2 = face_general<T, kRel>(Rc, Rc, -1, 1, dy, rdy, &br);
+          else G2 = face_general<T, kRel>(Lc, Rc, 0, 1, dy, rdy, &br);
+        }
+        Fs1 = {G2.h, G2.u, G2.v, G2.z};
       }
-      Fs1 = {F.h, F.t, F.n, F.z};
       if constexpr (KOUT) {
         if (own) {
           if (A.br_y) A.br_y[c * nx + x] = br;
@@ -964,7 +986,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     }
     R0 = R1; R1 = R2; R2 = Rn;
     if constexpr (!(QUICK && sizeof(T) == 8)) { qsh = qnh; qsu = qnu; qsv = qnv; qsz = qnz; }
-    if constexpr (EXT) { pp_uy = p_uy; pp_vy = p_vy; p_ux = sux; p_vx = svx; }
+    if constexpr (EXT) { pp_uy = p_uy; pp_vy = p_vy; }
+    if constexpr (EXT || NANBED) { p_ux = sux; p_vx = svx; }
     p_hx = shx; p_zx = szx; p_hy = shy; p_uy = suy; p_vy = svy; p_zy = szy;
     Fs0 = Fs1;
     Xw0 = Xw1;
