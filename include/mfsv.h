@@ -224,6 +224,26 @@ int mfsv_update_stage_field(mfsv_ctx* ctx);
  * of state 0 (U0), 1..nstages-1 (the intermediate RK rows of the last
  * timestep) or nstages (the last timestep's result, until it is accepted). */
 int mfsv_control_number(mfsv_ctx* ctx, int state, double dt, double* cn);
+/* Allocation-free control number.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same value as
+ * mfsv_control_number, bit for bit, of a state the caller holds in device
+ * memory: h, u, v are device arrays of nx*ny values (double for MFSV_F64,
+ * float for MFSV_F32) laid out as the context fields; dx, dy are the cell
+ * sizes.  The result dt * max_i((|u| + c)/dx + (|v| + c)/dy) is written to
+ * cn_out, a device pointer to one double, by work queued on `stream` (a
+ * hipStream_t; NULL = the null stream); the call does not synchronise.
+ * Allocates nothing: the block maxima go to a fixed array of 4096 doubles
+ * (32 KiB) in the library's device memory, one per GPU, held until the
+ * process ends.  Grids of more than MFSV_NOALLOC_MAX_CELLS cells return
+ * MFSV_ERR_ARG, as do bad sizes, spacings, dt or null pointers (before any
+ * device call).  Calls on one GPU share that array: calls that overlap on
+ * one GPU (concurrent streams or host threads) race and must be ordered by
+ * the caller. */
+#define MFSV_NOALLOC_MAX_CELLS ((int64_t)67108864) /* 4096 * 256 * 64 */
+int mfsv_control_number_noalloc(int precision, int device, int64_t nx, int64_t ny, double dx,
+                                double dy, const void* h, const void* u, const void* v,
+                                double dt, double* cn_out, void* stream);
 /* SpatialDerivativeOperator<T, Mesh, Cell, Minmod3>::apply<Axis>
  * (SpatialDerivative.hpp:48-53, 104-122; Minmod3.hpp:34-82) on an FP64 cell
  * field of the context's mesh: axis 0 = X (west, east), axis 1 = Y with the

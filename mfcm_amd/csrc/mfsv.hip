@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -476,6 +477,51 @@ __global__ void k_cn(const T* __restrict__ h, const T* __restrict__ u, const T* 
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// ---- allocation-free control number ------------------------------------
+// The block maxima of k_cn live in a file-scope device array instead of a
+// context's buffer: one copy per GPU, held until the process ends.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in there).
+constexpr int kNoAllocThreads = 256;
+constexpr int64_t kNoAllocCellsPerBlock = int64_t(kNoAllocThreads) * 64;
+constexpr int kNoAllocParts = 4096;
+constexpr int kNoAllocDevices = 64;
+static_assert(int64_t(kNoAllocParts) * kNoAllocCellsPerBlock == MFSV_NOALLOC_MAX_CELLS,
+              "the header's cell limit is what the scratch array holds");
+static __device__ __align__(256) double scratch[kNoAllocParts];
+
+// dt times the maximum of n block maxima (each >= 0: k_cn skips NaN), as the
+// host does for mfsv_control_number (max is exact, so the grouping of the
+// cells into blocks does not change the result).
+__global__ void k_cn_final(const double* __restrict__ part, int n, double dt, double* __restrict__ out) {
+  __shared__ double sh[kNoAllocThreads];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = part[i] > m ? part[i] : m;
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kNoAllocThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = dt * sh[0];
+}
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current.
+std::mutex g_scratch_mu;
+double* g_scratch[kNoAllocDevices] = {};
+
+int scratch_on(int device, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MFSV_ERR_HIP;
+    g_scratch[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch[device];
+  return MFSV_OK;
+}
+
 // SpatialDerivativeOperationKernel with Minmod3 (SpatialDerivative.hpp:104-122,
 // Minmod3.hpp:34-82) on a FP64 cell field: axis 0 takes (west, east) as
 // (left, right), axis 1 the reference's (north, south).  A missing neighbour
@@ -936,6 +982,34 @@ int mfsv_control_number(mfsv_ctx* c, int state, double dt, double* cn) {
   for (double x : hp) m = x > m ? x : m;
   *cn = dt * m;
   return MFSV_OK;
+}
+
+int mfsv_control_number_noalloc(int precision, int device, int64_t nx, int64_t ny, double dx,
+                                double dy, const void* h, const void* u, const void* v, double dt,
+                                double* cn_out, void* stream) {
+  if (precision != MFSV_F64 && precision != MFSV_F32) return MFSV_ERR_ARG;
+  if (device < 0 || device >= kNoAllocDevices || nx < 1 || ny < 1) return MFSV_ERR_ARG;
+  if (nx > MFSV_NOALLOC_MAX_CELLS || ny > MFSV_NOALLOC_MAX_CELLS / nx) return MFSV_ERR_ARG;
+  if (!(dx > 0) || !(dy > 0) || !(dt > 0) || !h || !u || !v || !cn_out) return MFSV_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return MFSV_ERR_HIP;
+  double* part = nullptr;
+  int rc = scratch_on(device, &part);
+  if (rc) return rc;
+  const int64_t n = nx * ny;
+  const int blocks = (int)((n + kNoAllocCellsPerBlock - 1) / kNoAllocCellsPerBlock);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == MFSV_F64)
+    hipLaunchKernelGGL(k_cn<double>, dim3(blocks), dim3(kNoAllocThreads), 0, s,
+                       static_cast<const double*>(h), static_cast<const double*>(u),
+                       static_cast<const double*>(v), n, dx, 1.0 / dx, dy, 1.0 / dy, part);
+  else
+    hipLaunchKernelGGL(k_cn<float>, dim3(blocks), dim3(kNoAllocThreads), 0, s,
+                       static_cast<const float*>(h), static_cast<const float*>(u),
+                       static_cast<const float*>(v), n, float(dx), float(1.0 / dx), float(dy),
+                       float(1.0 / dy), part);
+  hipLaunchKernelGGL(k_cn_final, dim3(1), dim3(kNoAllocThreads), 0, s, (const double*)part, blocks,
+                     dt, cn_out);
+  return hipGetLastError() == hipSuccess ? MFSV_OK : MFSV_ERR_HIP;
 }
 
 int mfsv_spatial_derivative(mfsv_ctx* c, int axis, const double* in, double* out, int64_t count) {

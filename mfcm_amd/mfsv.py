@@ -119,6 +119,7 @@ SIGNATURES = {
     "mfsv_diagnose": (_I, [_P, _I, _D, _D, _I]),
     "mfsv_update_stage_field": (_I, [_P]),
     "mfsv_control_number": (_I, [_P, _I, _D, C.POINTER(C.c_double)]),
+    "mfsv_control_number_noalloc": (_I, [_I, _I, _I64, _I64, _D, _D, _P, _P, _P, _D, _P, _P]),
     "mfsv_spatial_derivative": (_I, [_P, _I, _P, _P, _I64]),
     "mfsv_timestep": (_I, [_P, _D, _D, C.POINTER(C.c_double)]),
     "mfsv_accept": (_I, [_P]),

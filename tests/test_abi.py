@@ -59,3 +59,21 @@ def test_grid_limits_without_device():
         assert L.mfsv_create(C.byref(bad), mfsv.F64, 0, C.byref(h)) == -1
     assert L.mfsv_create(C.byref(mfsv.Grid(5, 5, 1.0, 1.0, 0.0, 0.0)), 7, 0, C.byref(h)) == -1
     assert not h.value
+
+
+def test_noalloc_control_number_rejects_bad_arguments_without_device():
+    # sizes, spacings, dt, precision and null pointers are checked before any
+    # device call; the cell limit is what the fixed scratch array holds
+    import ctypes as C
+    L = mfsv.lib()
+    p = C.c_void_p(8)
+    f = L.mfsv_control_number_noalloc
+    assert f(mfsv.F64, 0, 8193, 8192, 1.0, 1.0, p, p, p, 0.1, p, None) == -1   # > 2^26 cells
+    assert f(mfsv.F32, 0, 1 << 40, 1 << 40, 1.0, 1.0, p, p, p, 0.1, p, None) == -1
+    assert f(mfsv.F64, 0, 0, 5, 1.0, 1.0, p, p, p, 0.1, p, None) == -1
+    assert f(mfsv.F64, 0, 5, 5, 0.0, 1.0, p, p, p, 0.1, p, None) == -1
+    assert f(mfsv.F64, 0, 5, 5, 1.0, 1.0, p, p, p, 0.0, p, None) == -1
+    assert f(mfsv.F64, 0, 5, 5, 1.0, 1.0, None, p, p, 0.1, p, None) == -1
+    assert f(mfsv.F64, 0, 5, 5, 1.0, 1.0, p, p, p, 0.1, None, None) == -1
+    assert f(7, 0, 5, 5, 1.0, 1.0, p, p, p, 0.1, p, None) == -1
+    assert f(mfsv.F64, -1, 5, 5, 1.0, 1.0, p, p, p, 0.1, p, None) == -1
