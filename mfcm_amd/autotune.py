@@ -22,7 +22,13 @@ import os
 CANDIDATES = (4, 6, 8, 12, 14, 15, 16, 18, 20, 24, 28, 32, 36, 40, 48)
 STRIP = 60
 SIMDS = 1024          # 256 CUs x 4 SIMDs on MI355X
-OCCUPANCY = {0: 2, 1: 4}   # waves per SIMD of the stage kernel: F64, F32
+# Waves per SIMD of the stage kernels at their register counts (F64, F32):
+# the single-catchment FP64 kernel of power-of-two spacing is built for 3
+# (<= 168 VGPRs), the batch kernel holds 2 FP64 / 4 FP32 waves (the batch's
+# persistent grid asks the runtime for the same figure).
+OCCUPANCY = {0: 3, 1: 4}
+OCCUPANCY_BATCH = {0: 2, 1: 4}
+REPEATS = 3           # interleaved timing rounds per candidate (median)
 
 
 def model_rows(nx: int, ny: int, precision: int) -> int:
@@ -46,7 +52,7 @@ def model_rows_batch(shapes, precision: int) -> int:
     launch: the same trade-off over the concatenated tile table, where the
     halo overhead is weighted by each grid's cells and the last strip of a
     grid wastes the rows beyond ny."""
-    occ = OCCUPANCY.get(precision, 2)
+    occ = OCCUPANCY_BATCH.get(precision, 2)
     cells = sum(nx * ny for nx, ny in shapes)
     if cells == 0:
         return CANDIDATES[0]
@@ -78,30 +84,35 @@ class TileTuner:
     def rows_for(self, nx, ny, precision) -> int:
         return self.table.get(self.key(precision, nx, ny), model_rows(nx, ny, precision))
 
-    def tune(self, scenario, precision, warmup=20, samples=20, candidates=CANDIDATES) -> int:
-        """Time every candidate on a scratch catchment of the same scenario."""
+    def tune(self, scenario, precision, warmup=20, samples=20, candidates=CANDIDATES,
+             repeats=REPEATS) -> int:
+        """Time every candidate on a scratch catchment of the same scenario:
+        the state after the warmup is restored before each timing, the
+        candidates are timed in `repeats` interleaved rounds, and each one's
+        median stage-launch time decides."""
+        import statistics
+
         from .mfsv import Catchment
         sc = copy.copy(scenario)
         cat = Catchment(sc, precision=precision)
         try:
             cat.run_begin()
             cat.run(warmup, graph=True)
-            best, best_us = None, float("inf")
-            timings = {}
-            for r in candidates:
-                cat.set_tile_rows(r)
-                cat.run(samples, graph=2)
-                ms, n = cat.last_stage_ms()
-                if n == 0:
-                    continue
-                us = ms / n * 1e3
-                timings[r] = us
-                if us < best_us:
-                    best, best_us = r, us
+            snap = cat.state()
+            times = {r: [] for r in candidates}
+            for _ in range(repeats):
+                for r in candidates:
+                    cat.set_state(*snap)
+                    cat.set_tile_rows(r)
+                    cat.run(samples, graph=2)
+                    ms, n = cat.last_stage_ms()
+                    if n:
+                        times[r].append(ms / n * 1e3)
+            timings = {r: statistics.median(v) for r, v in times.items() if v}
         finally:
             cat.close()
-        if best is None:
-            best = model_rows(scenario.nx, scenario.ny, precision)
+        best = min(timings, key=timings.get) if timings else model_rows(
+            scenario.nx, scenario.ny, precision)
         self.table[self.key(precision, scenario.nx, scenario.ny)] = best
         self.last_timings = timings
         return best
@@ -111,31 +122,36 @@ class TileTuner:
         return self.table.get(key, model_rows_batch(shapes, precision))
 
     def tune_batch(self, scenarios, precision, warmup=16, samples=48,
-                   candidates=CANDIDATES) -> int:
+                   candidates=CANDIDATES, repeats=REPEATS) -> int:
         """Time a shared tile height for a batch: scratch copies of the members
-        run `samples` timesteps per candidate (the batch's event-timed run).
-        The members advance while they are timed, so a candidate is timed on
-        a state a little later than the previous one; the scratch states are
-        discarded."""
+        run `samples` timesteps per candidate (the batch's event-timed run),
+        each timing from the members' states after the warmup, in `repeats`
+        interleaved rounds; the median time per timestep decides."""
+        import statistics
+
         from .batch import Batch
         from .mfsv import Catchment
         cats = [Catchment(copy.copy(sc), precision=precision) for sc in scenarios]
         b = Batch(precision)
-        timings = {}
+        times = {r: [] for r in candidates}
         try:
             for c in cats:
                 c.run_begin()
                 b.add(c)
             b.run(warmup)
-            for r in candidates:
-                for c in cats:
-                    c.set_tile_rows(r)
-                b.run(samples)
-                timings[r] = b.last_run_ms() / samples * 1e3
+            snaps = [c.state() for c in cats]
+            for _ in range(repeats):
+                for r in candidates:
+                    for c, sn in zip(cats, snaps):
+                        c.set_state(*sn)
+                        c.set_tile_rows(r)
+                    b.run(samples)
+                    times[r].append(b.last_run_ms() / samples * 1e3)
         finally:
             b.close()
             for c in cats:
                 c.close()
+        timings = {r: statistics.median(v) for r, v in times.items() if v}
         shapes = [(sc.nx, sc.ny) for sc in scenarios]
         best = min(timings, key=timings.get) if timings else model_rows_batch(shapes, precision)
         key = f"{int(precision)}:batch{len(shapes)}:{sum(nx * ny for nx, ny in shapes)}"

@@ -211,9 +211,18 @@ int mfsv_batch_create(int precision, int device, mfsv_batch** out) {
     mfsv_batch_destroy(b);
     return MFSV_ERR_HIP;
   }
-  // persistent grid: one block of 4 waves per SIMD set, times the waves a
-  // SIMD holds at the stage kernel's register count
-  b->nblocks = prop.multiProcessorCount * (precision == MFSV_F64 ? 2 : 4);
+  // persistent grid: as many blocks per CU as the batch stage kernel's
+  // register count lets the CU hold (the runtime's occupancy query on the
+  // Heun-2 first-stage variant; 2 blocks = 2 waves per SIMD for FP64, 4 for
+  // FP32 with the current register counts)
+  int per_cu = 0;
+  hipError_t oe = precision == MFSV_F64
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_stage_batch<double, 2, true, false, false>, kWave * kWavesPerBlock, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_stage_batch<float, 2, true, false, false>, kWave * kWavesPerBlock, 0);
+  if (oe != hipSuccess || per_cu < 1) per_cu = precision == MFSV_F64 ? 2 : 4;
+  b->nblocks = prop.multiProcessorCount * per_cu;
   *out = b;
   return MFSV_OK;
 }

@@ -28,7 +28,7 @@ import time
 import numpy as np
 
 from . import scenarios as S
-from .autotune import TileTuner
+from .autotune import TileTuner, model_rows_batch
 from .batch import Batch
 from .mfsv import F32, Catchment
 
@@ -84,6 +84,9 @@ class FloodServer:
         self.device = device
         self.batch = Batch(precision, device)
         self.used = 0.0
+        self.peak_used = 0.0      # largest device footprint resident at once
+        self.peak_active = 0
+        self.held_back = set()    # requests that waited for HBM room
         self.active = {}          # rid -> (request, catchment, bytes)
         self.done = []
         self.refused = []
@@ -101,12 +104,19 @@ class FloodServer:
             self.refused.append(req)
             return True
         if self.used + need > self.budget:
+            self.held_back.add(req.rid)   # arrived, ready, but no room yet
             return False
         cat = Catchment(sc, precision=self.precision, device=self.device)
-        cat.set_tile_rows(self.tuner.rows_for(sc.nx, sc.ny, self.precision))
+        # the members share one persistent launch: the tile height that suits
+        # the resident set with the new member (fewer halo rows once the
+        # batch fills the chip; results do not depend on it)
+        shapes = [(c.nx, c.ny) for _, c, _, _, _ in self.active.values()] + [(sc.nx, sc.ny)]
+        cat.set_tile_rows(model_rows_batch(shapes, self.precision))
         cat.run_begin()
         self.batch.add(cat)
         self.used += need
+        self.peak_used = max(self.peak_used, self.used)
+        self.peak_active = max(self.peak_active, len(self.active) + 1)
         self.prepared.pop(req.rid, None)
         self.active[req.rid] = (req, cat, need, now, sc.ncells)
         return True
