@@ -271,12 +271,12 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc)
   return A;
 }
 
-template <typename T, bool QUICK, bool NANBED, bool EXT>
+template <typename T, bool QUICK, bool NANBED, bool EXT, bool BND>
 void launch_q(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
   dim3 grid((unsigned)c->nblocks), block(kWave * kWavesPerBlock);
   const bool base = A.stage > 0;
 #define MFSV_LAUNCH(N, K, B) \
-  hipLaunchKernelGGL((k_stage<T, N, K, QUICK, NANBED, EXT, B>), grid, block, 0, c->stream, A)
+  hipLaunchKernelGGL((k_stage<T, N, K, QUICK, NANBED, EXT, B, BND>), grid, block, 0, c->stream, A)
   if (kout) { MFSV_LAUNCH(0, true, false); return; }
   switch (nacc) {
     case 1: if (base) MFSV_LAUNCH(1, false, true); else MFSV_LAUNCH(1, false, false); break;
@@ -293,10 +293,24 @@ bool has_ext(const mfsv_ctx* c) {
          (c->P.manning && c->ndpf.p && c->dshf.p && c->ddpf.p);
 }
 
+// Per-cell discharge or head endpoint fields in use (the FP32 form loads
+// them with the row only when they are).
+bool has_bnd(const mfsv_ctx* c) { return c->P.discharge == 2 || c->P.head == 2; }
+
+template <typename T, bool QUICK, bool NANBED, bool EXT>
+void launch_b(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
+  if constexpr (sizeof(T) == 4) {
+    if (has_bnd(c)) launch_q<T, QUICK, NANBED, EXT, true>(c, A, nacc, kout);
+    else launch_q<T, QUICK, NANBED, EXT, false>(c, A, nacc, kout);
+  } else {
+    launch_q<T, QUICK, NANBED, EXT, false>(c, A, nacc, kout);
+  }
+}
+
 template <typename T, bool QUICK, bool NANBED>
 void launch_x(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
-  if (has_ext(c)) launch_q<T, QUICK, NANBED, true>(c, A, nacc, kout);
-  else launch_q<T, QUICK, NANBED, false>(c, A, nacc, kout);
+  if (has_ext(c)) launch_b<T, QUICK, NANBED, true>(c, A, nacc, kout);
+  else launch_b<T, QUICK, NANBED, false>(c, A, nacc, kout);
 }
 
 // True when x is a power of two: x/d == x * (1/d) exactly.

@@ -27,7 +27,7 @@ struct mfsv_batch {
   int ntiles = 0;
   bool dirty = true;
   bool quick = true;
-  bool any_nanbed = false;         // a member has coded-out cells or further sources
+  bool any_nanbed = false;         // a member needs the general stage path
   int nblocks = 0;                 // persistent grid
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -114,8 +114,11 @@ int upload_tables(mfsv_batch* b) {
   bool quick = true;  // exact division stays exact for power-of-two spacing
   if (sizeof(T) == 8)
     for (auto* c : b->active) quick = quick && pow2(c->dx) && pow2(c->dy);
-  bool any_nanbed = false;  // a member needs the coded-out or further-sources path
-  for (auto* c : b->active) any_nanbed = any_nanbed || c->nanbed || has_ext(c);
+  // a member needs the general path: coded-out cells, further sources, or
+  // (FP32) per-cell discharge / head endpoint fields
+  bool any_nanbed = false;
+  for (auto* c : b->active)
+    any_nanbed = any_nanbed || c->nanbed || has_ext(c) || (sizeof(T) == 4 && has_bnd(c));
   if (quick != b->quick || any_nanbed != b->any_nanbed) bdrop_graph(b);
   b->quick = quick;
   b->any_nanbed = any_nanbed;

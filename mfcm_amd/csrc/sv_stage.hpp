@@ -236,7 +236,10 @@ __device__ __forceinline__ Side<T> outflow_ghost(const Side<T>& c) {
 // and the stage boundary are compiled in (each still switched at run time).
 // BASE: a later stage (s > 0), whose accumulator rows are read back as the
 // bases of this stage's updates; the first stage updates from its input.
-template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE>
+// BND: per-cell discharge / head endpoint fields (discharge or head mode 2)
+// may be set; without it the FP32 form neither loads nor holds their
+// pointers (FP64 reads them where they are used, whatever BND says).
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE, bool BND>
 __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile, const int lane) {
   static_assert(!(QUICK && NANBED), "coded-out cells take the quotient path");
   constexpr bool kRel = sizeof(T) == 4;
@@ -410,7 +413,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   // the words the row's low-order depth load has just fetched (the bed where
   // there is none); its value is not used.  FP64 loads them where they are
   // used.
-  constexpr bool kPreBnd = sizeof(T) == 4;
+  constexpr bool kPreBnd = sizeof(T) == 4 && BND;
+  constexpr bool kBnd = sizeof(T) == 8 || BND;  // per-cell endpoint fields possible
   const T* const fb = (sizeof(T) == 4 && !KOUT) ? hl_in : zb;
   const T* const q0p = A.discharge == 2 ? A.q0f : fb;
   const T* const q1p = A.discharge == 2 ? A.q1f : fb;
@@ -847,7 +851,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       // DischargeBoundarySourceTerm.cpp:43-67 (E1 flips the guard)
       if (A.discharge || storm) {
         T tq0, tq1;
-        if (A.discharge == 2) {
+        if (kBnd && A.discharge == 2) {
           if constexpr (kPreBnd) { tq0 = X0.q0; tq1 = X0.q1; }
           else { tq0 = ld(A.q0f, wo); tq1 = ld(A.q1f, wo); }
         }
@@ -876,7 +880,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       // HeadBoundarySourceTerm.cpp:43-66 (overwrites dh)
       if (A.head) {
         T h0, h1;
-        if (A.head == 2) {
+        if (kBnd && A.head == 2) {
           if constexpr (kPreBnd) { h0 = X0.hb0; h1 = X0.hb1; }
           else { h0 = ld(A.hb0f, wo); h1 = ld(A.hb1f, wo); }
         }
@@ -1023,7 +1027,7 @@ template <typename T, bool QUICK, bool NANBED, bool EXT>
 constexpr int stage_waves() {
   return sizeof(T) == 4 ? 4 : ((QUICK && !NANBED && !EXT) ? 3 : 2);
 }
-template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE>
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE, bool BND>
 __global__ __launch_bounds__(kWave * kWavesPerBlock)
 __attribute__((amdgpu_waves_per_eu(stage_waves<T, QUICK, NANBED, EXT>()))) void k_stage(const StageArgs<T> A) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -1031,7 +1035,7 @@ __attribute__((amdgpu_waves_per_eu(stage_waves<T, QUICK, NANBED, EXT>()))) void 
   const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
                                                   (int)(threadIdx.x >> 6));
   if (tile >= A.ntiles) return;
-  stage_tile<T, NACC, KOUT, QUICK, NANBED, EXT, BASE>(A, tile, lane);
+  stage_tile<T, NACC, KOUT, QUICK, NANBED, EXT, BASE, BND>(A, tile, lane);
 }
 
 // Many catchments in one launch: a persistent grid walks a tile table of
@@ -1068,15 +1072,15 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
     if constexpr (GENERAL) {
       const bool ext = (a.eloss | a.infil | a.stageb | a.eddy) != 0;
       if (a.nanbed) {
-        stage_tile<T, NACC, false, false, true, true, BASE>(a, t, lane);
+        stage_tile<T, NACC, false, false, true, true, BASE, true>(a, t, lane);
         continue;
       }
       if (ext) {
-        stage_tile<T, NACC, false, QUICK, false, true, BASE>(a, t, lane);
+        stage_tile<T, NACC, false, QUICK, false, true, BASE, true>(a, t, lane);
         continue;
       }
     }
-    stage_tile<T, NACC, false, QUICK, false, false, BASE>(a, t, lane);
+    stage_tile<T, NACC, false, QUICK, false, false, BASE, GENERAL>(a, t, lane);
   }
 }
 
