@@ -293,18 +293,14 @@ bool has_ext(const mfsv_ctx* c) {
          (c->P.manning && c->ndpf.p && c->dshf.p && c->ddpf.p);
 }
 
-// Per-cell discharge or head endpoint fields in use (the FP32 form loads
-// them with the row only when they are).
+// Per-cell discharge or head endpoint fields in use (the stage kernels
+// load them only when they are).
 bool has_bnd(const mfsv_ctx* c) { return c->P.discharge == 2 || c->P.head == 2; }
 
 template <typename T, bool QUICK, bool NANBED, bool EXT>
 void launch_b(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
-  if constexpr (sizeof(T) == 4) {
-    if (has_bnd(c)) launch_q<T, QUICK, NANBED, EXT, true>(c, A, nacc, kout);
-    else launch_q<T, QUICK, NANBED, EXT, false>(c, A, nacc, kout);
-  } else {
-    launch_q<T, QUICK, NANBED, EXT, false>(c, A, nacc, kout);
-  }
+  if (has_bnd(c)) launch_q<T, QUICK, NANBED, EXT, true>(c, A, nacc, kout);
+  else launch_q<T, QUICK, NANBED, EXT, false>(c, A, nacc, kout);
 }
 
 template <typename T, bool QUICK, bool NANBED>

@@ -115,10 +115,10 @@ int upload_tables(mfsv_batch* b) {
   if (sizeof(T) == 8)
     for (auto* c : b->active) quick = quick && pow2(c->dx) && pow2(c->dy);
   // a member needs the general path: coded-out cells, further sources, or
-  // (FP32) per-cell discharge / head endpoint fields
+  // per-cell discharge / head endpoint fields
   bool any_nanbed = false;
   for (auto* c : b->active)
-    any_nanbed = any_nanbed || c->nanbed || has_ext(c) || (sizeof(T) == 4 && has_bnd(c));
+    any_nanbed = any_nanbed || c->nanbed || has_ext(c) || has_bnd(c);
   if (quick != b->quick || any_nanbed != b->any_nanbed) bdrop_graph(b);
   b->quick = quick;
   b->any_nanbed = any_nanbed;

@@ -114,9 +114,11 @@ __device__ __forceinline__ double clamp_t(double x, double lo, double hi) {
 }
 // max(x, 0): one median instruction in FP32 (fmax of a value that the
 // compiler cannot prove canonical costs a second instruction that quiets a
-// signalling NaN first); a NaN gives 0 in both forms.
+// signalling NaN first; the upper bound is the largest finite float, since
+// the compiler turns a median with +inf back into that fmax).  Equal to
+// max(x, 0) for every finite x; a NaN gives 0 in both forms.
 __device__ __forceinline__ float max0(float x) {
-  return __builtin_amdgcn_fmed3f(x, 0.0f, __builtin_huge_valf());
+  return __builtin_amdgcn_fmed3f(x, 0.0f, 3.40282347e+38f);
 }
 __device__ __forceinline__ double max0(double x) { return __builtin_fmax(x, 0.0); }
 // Sign bits differ.

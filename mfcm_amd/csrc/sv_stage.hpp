@@ -237,8 +237,8 @@ __device__ __forceinline__ Side<T> outflow_ghost(const Side<T>& c) {
 // BASE: a later stage (s > 0), whose accumulator rows are read back as the
 // bases of this stage's updates; the first stage updates from its input.
 // BND: per-cell discharge / head endpoint fields (discharge or head mode 2)
-// may be set; without it the FP32 form neither loads nor holds their
-// pointers (FP64 reads them where they are used, whatever BND says).
+// may be set; without it the kernel neither loads nor holds their pointers
+// (the FP32 form loads them with the row, FP64 where they are used).
 template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE, bool BND>
 __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile, const int lane) {
   static_assert(!(QUICK && NANBED), "coded-out cells take the quotient path");
@@ -414,7 +414,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   // there is none); its value is not used.  FP64 loads them where they are
   // used.
   constexpr bool kPreBnd = sizeof(T) == 4 && BND;
-  constexpr bool kBnd = sizeof(T) == 8 || BND;  // per-cell endpoint fields possible
+  constexpr bool kBnd = BND;  // per-cell endpoint fields possible
   const T* const fb = (sizeof(T) == 4 && !KOUT) ? hl_in : zb;
   const T* const q0p = A.discharge == 2 ? A.q0f : fb;
   const T* const q1p = A.discharge == 2 ? A.q1f : fb;
@@ -478,8 +478,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   double vol = 0.0, rate = 0.0;
   T qmax = T(0);
 
-#pragma unroll 4
-  for (int c = y0 - 1; c <= y1; ++c) {
+  // One row of the march: slopes of row c, the west x-faces of row c, the
+  // south y-faces of row c and the cell pass of row c - 1.
+  auto row_step = [&](const int c) __attribute__((always_inline)) {
     // inputs of row c-1's cell pass: FP32 loads them here (consumed after the
     // slopes and faces; fewer live registers) and before the row prefetch,
     // so that waiting for them leaves the prefetch in flight; FP64 one row
@@ -618,7 +619,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     }
     // ---- south y-face of row c (face id (nx+1)ny + c*nx + x)
     FaceUV<T> Fs1 = {};
-    if (c >= y0) {
+    if (c >= y0 && c <= y1) {
       float br;
       Flux<T> F;
       if constexpr (!NANBED) {
@@ -995,6 +996,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     p_hx = shx; p_zx = szx; p_hy = shy; p_uy = suy; p_vy = svy; p_zy = szy;
     Fs0 = Fs1;
     Xw0 = Xw1;
+  };
+  // FP32: two rows per trip (the loop holds cross-lane operations, which the
+  // compiler will not unroll with a remainder), so that the row registers are
+  // renamed instead of copied.  An odd row count runs one extra step at
+  // c = y1 + 1, which computes no face or cell of the tile and stores nothing
+  // (its loads are clamped into the grid).  FP64 keeps one row per trip: its
+  // register budget at 3 waves per SIMD has no room for the second row.
+  if constexpr (sizeof(T) == 4) {
+    for (int c = y0 - 1; c <= y1; c += 2) {
+      row_step(c);
+      row_step(c + 1);
+    }
+  } else {
+    for (int c = y0 - 1; c <= y1; ++c) row_step(c);
   }
 
   // the lanes that own a cell, then wave reductions (fixed butterfly order ->
