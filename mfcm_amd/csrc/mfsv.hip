@@ -203,6 +203,13 @@ int get_split(mfsv_ctx* c, const Buf& hi, const Buf& lo, double* host, int64_t n
 template <typename T>
 T* tp(const Buf& b) { return static_cast<T*>(b.p); }
 
+// The FP32 form advances Heun-2 in its convex form (a first stage that
+// writes y1 only; see stage_tile's CONVEX).
+bool convex_rk(const mfsv_ctx* c) {
+  return c->prec == MFSV_F32 && c->nstages == 2 && c->tab[1][0] == 1.0 && c->tab[1][1] == 1.0 &&
+         c->tab[2][0] == 0.0 && c->tab[2][1] == 0.5 && c->tab[2][2] == 0.5;
+}
+
 template <typename T>
 StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc) {
   StageArgs<T> A{};
@@ -253,8 +260,11 @@ StageArgs<T> stage_args(mfsv_ctx* c, int s, bool kout, bool branches, int* nacc)
   if (kout) {
     A.k_h = tp<T>(c->kf[0]); A.k_u = tp<T>(c->kf[1]); A.k_v = tp<T>(c->kf[2]);
   } else {
-    // rows i = s+1 .. n receive k_s * a(i, s+1)  (RungeKutta.hpp:377-388)
-    *nacc = c->nstages - s;
+    // rows i = s+1 .. n receive k_s * a(i, s+1)  (RungeKutta.hpp:377-388);
+    // the convex form's first stage feeds row 1 only, which the second
+    // stage completes (E8/E9) as it reads it
+    *nacc = (convex_rk(c) && s == 0) ? 1 : c->nstages - s;
+    A.adj_in = (convex_rk(c) && s == 1) ? 1 : 0;
     for (int j = 0; j < *nacc; ++j) {
       const int i = s + 1 + j;
       A.acoef[j] = c->tab[i][s + 1];
@@ -276,8 +286,20 @@ void launch_q(mfsv_ctx* c, const StageArgs<T>& A, int nacc, bool kout) {
   dim3 grid((unsigned)c->nblocks), block(kWave * kWavesPerBlock);
   const bool base = A.stage > 0;
 #define MFSV_LAUNCH(N, K, B) \
-  hipLaunchKernelGGL((k_stage<T, N, K, QUICK, NANBED, EXT, B, BND>), grid, block, 0, c->stream, A)
+  hipLaunchKernelGGL((k_stage<T, N, K, QUICK, NANBED, EXT, B, BND, false>), grid, block, 0, \
+                     c->stream, A)
   if (kout) { MFSV_LAUNCH(0, true, false); return; }
+  if constexpr (sizeof(T) == 4) {
+    if (convex_rk(c)) {  // one row per stage
+      if (base)
+        hipLaunchKernelGGL((k_stage<T, 1, false, QUICK, NANBED, EXT, true, BND, true>), grid,
+                           block, 0, c->stream, A);
+      else
+        hipLaunchKernelGGL((k_stage<T, 1, false, QUICK, NANBED, EXT, false, BND, true>), grid,
+                           block, 0, c->stream, A);
+      return;
+    }
+  }
   switch (nacc) {
     case 1: if (base) MFSV_LAUNCH(1, false, true); else MFSV_LAUNCH(1, false, false); break;
     case 2: if (base) MFSV_LAUNCH(2, false, true); else MFSV_LAUNCH(2, false, false); break;
@@ -343,6 +365,8 @@ void launch_diag_typed(mfsv_ctx* c, int s, bool prev_u0, int what) {
     A.in_h = tp<T>(c->U[o][0]); A.in_u = tp<T>(c->U[o][1]); A.in_v = tp<T>(c->U[o][2]);
   }
   A.diag = 1;
+  // the convex form's y1 (the last stage's input) is stored before E8/E9
+  A.adj_in = (!prev_u0 && s == 1 && convex_rk(c)) ? 1 : 0;
   // the output fields depend on the state, the bed and the Manning
   // parameters only: no boundary or further source field is read
   A.discharge = 0; A.head = 0; A.stageb = 0;
@@ -467,15 +491,23 @@ int compute_volume(mfsv_ctx* c, int which, double* out) {
 // c = sqrt(9.81 max(h, 0)) (quotients correctly rounded); the caller takes
 // the maximum of the block maxima and multiplies by dt (max_i dt q_i ==
 // dt max_i q_i for dt > 0).
+// adj != 0: the state is a convex-form y1 stored before E8/E9 (flags,
+// h_dry, u_max), which apply to its velocities first.
 template <typename T>
 __global__ void k_cn(const T* __restrict__ h, const T* __restrict__ u, const T* __restrict__ v,
-                     int64_t n, T dx, T rdx, T dy, T rdy, double* part) {
+                     int64_t n, T dx, T rdx, T dy, T rdy, double* part, int adj = 0,
+                     int flags = 0, T h_dry = T(0), T u_max = T(0)) {
   __shared__ double sh[256];
   double m = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
+    T ui = u[i], vi = v[i];
+    if (adj) {
+      if ((flags & 0x80) && h[i] <= h_dry) { ui = T(0); vi = T(0); }
+      if (flags & 0x100) { ui = clamp_t(ui, -u_max, u_max); vi = clamp_t(vi, -u_max, u_max); }
+    }
     const T c = fsqrt_t(T(kG) * fmax_t(h[i], T(0)));
-    const T q = div_c(fabs_t(u[i]) + c, dx, rdx) + div_c(fabs_t(v[i]) + c, dy, rdy);
+    const T q = div_c(fabs_t(ui) + c, dx, rdx) + div_c(fabs_t(vi) + c, dy, rdy);
     m = (double(q) > m) ? double(q) : m;
   }
   sh[threadIdx.x] = m;
@@ -982,7 +1014,9 @@ int mfsv_control_number(mfsv_ctx* c, int state, double dt, double* cn) {
   else
     hipLaunchKernelGGL(k_cn<float>, dim3(kVolBlocks), dim3(256), 0, c->stream, tp<float>(f[0]),
                        tp<float>(f[1]), tp<float>(f[2]), c->nc, float(c->dx), float(1.0 / c->dx),
-                       float(c->dy), float(1.0 / c->dy), part);
+                       float(c->dy), float(1.0 / c->dy), part,
+                       (state > 0 && state < c->nstages && convex_rk(c)) ? 1 : 0, c->P.flags,
+                       float(c->P.h_dry), float(c->P.u_max));
   HIPCHK(c, hipGetLastError());
   std::vector<double> hp(kVolBlocks);
   HIPCHK(c, hipMemcpyAsync(hp.data(), part, sizeof(double) * kVolBlocks, hipMemcpyDeviceToHost,

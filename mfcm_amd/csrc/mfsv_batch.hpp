@@ -28,6 +28,7 @@ struct mfsv_batch {
   bool dirty = true;
   bool quick = true;
   bool any_nanbed = false;         // a member needs the general stage path
+  bool convex = false;             // FP32 Heun-2: the convex form (convex_rk)
   int nblocks = 0;                 // persistent grid
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -122,6 +123,10 @@ int upload_tables(mfsv_batch* b) {
   if (quick != b->quick || any_nanbed != b->any_nanbed) bdrop_graph(b);
   b->quick = quick;
   b->any_nanbed = any_nanbed;
+  // the members share the scheme and precision: the first one decides
+  const bool cvx = !b->active.empty() && convex_rk(b->active[0]);
+  if (cvx != b->convex) bdrop_graph(b);
+  b->convex = cvx;
   for (int s = 0; s < b->nstages; ++s) {
     std::vector<StageArgs<T>> a(n);
     for (int j = 0; j < n; ++j) {
@@ -159,8 +164,20 @@ void launch_batch_stage(mfsv_batch* b, int s) {
   const int2* t = static_cast<const int2*>(b->tiles.p);
   dim3 grid((unsigned)b->nblocks), block(kWave * kWavesPerBlock);
   const bool base = s > 0;
+  if constexpr (sizeof(T) == 4) {
+    if (b->convex) {  // FP32 Heun-2 in the convex form: one row per stage
+      if (base)
+        hipLaunchKernelGGL((k_stage_batch<T, 1, QUICK, GENERAL, true, true>), grid, block, 0,
+                           b->stream, m, t, cnt);
+      else
+        hipLaunchKernelGGL((k_stage_batch<T, 1, QUICK, GENERAL, false, true>), grid, block, 0,
+                           b->stream, m, t, cnt);
+      return;
+    }
+  }
 #define MFSV_BLAUNCH(N, B) \
-  hipLaunchKernelGGL((k_stage_batch<T, N, QUICK, GENERAL, B>), grid, block, 0, b->stream, m, t, cnt)
+  hipLaunchKernelGGL((k_stage_batch<T, N, QUICK, GENERAL, B, false>), grid, block, 0, b->stream, m, \
+                     t, cnt)
   switch (b->nstages - s) {
     case 1: if (base) MFSV_BLAUNCH(1, true); else MFSV_BLAUNCH(1, false); break;
     case 2: if (base) MFSV_BLAUNCH(2, true); else MFSV_BLAUNCH(2, false); break;
@@ -221,9 +238,9 @@ int mfsv_batch_create(int precision, int device, mfsv_batch** out) {
   int per_cu = 0;
   hipError_t oe = precision == MFSV_F64
       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_stage_batch<double, 2, true, false, false>, kWave * kWavesPerBlock, 0)
+            &per_cu, k_stage_batch<double, 2, true, false, false, false>, kWave * kWavesPerBlock, 0)
       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_stage_batch<float, 2, true, false, false>, kWave * kWavesPerBlock, 0);
+            &per_cu, k_stage_batch<float, 1, true, false, false, true>, kWave * kWavesPerBlock, 0);
   if (oe != hipSuccess || per_cu < 1) per_cu = precision == MFSV_F64 ? 2 : 4;
   b->nblocks = prop.multiProcessorCount * per_cu;
   *out = b;

@@ -112,6 +112,9 @@ struct StageArgs {
   // Manning n and friction slope, the bed slopes.  diag != 0 leaves the
   // infiltration capacity untouched.
   int32_t diag;
+  // the stage input is a convex-form y1 stored before E8/E9: apply them to
+  // the velocities as the rows are loaded (the completed state they stand for)
+  int32_t adj_in;
   T* dg_fh; T* dg_fu; T* dg_fv; T* dg_fz;
   T* dg_nh; T* dg_sf; T* dg_zx; T* dg_zy;
 };
@@ -239,8 +242,15 @@ __device__ __forceinline__ Side<T> outflow_ghost(const Side<T>& c) {
 // BND: per-cell discharge / head endpoint fields (discharge or head mode 2)
 // may be set; without it the kernel neither loads nor holds their pointers
 // (the FP32 form loads them with the row, FP64 where they are used).
-template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE, bool BND>
+// CONVEX (FP32 form, Heun-2 only, NACC == 1): the stages in the convex form
+// y1 = y0 + k0 dt, y2 = (y0 + y1)/2 + k1 dt/2 (the same method; FP32 is not
+// held to the reference's summation order): the first stage writes y1 only,
+// the second reads y0 (with its low-order depth) and y1 and writes y2 with a
+// compensated depth, so no partial row y0 + k0 dt/2 is stored and read back.
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE, bool BND,
+          bool CONVEX = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile, const int lane) {
+  static_assert(!CONVEX || (sizeof(T) == 4 && NACC == 1 && !KOUT), "convex form: FP32 Heun-2");
   static_assert(!(QUICK && NANBED), "coded-out cells take the quotient path");
   constexpr bool kRel = sizeof(T) == 4;
   constexpr uint32_t S = sizeof(T);
@@ -378,9 +388,27 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     }
   }
 
+  // E8 / E9 on a completed state's velocities (no velocity where h <= h_dry,
+  // components within +-u_max)
+  auto complete = [&](T hh, T& uu, T& vv) {
+    if (A.flags & 0x80) {  // E8
+      const bool dryc = hh <= A.h_dry;
+      uu = dryc ? T(0) : uu;
+      vv = dryc ? T(0) : vv;
+    }
+    if (A.flags & 0x100) {  // E9
+      uu = clamp_t(uu, -A.u_max, A.u_max);
+      vv = clamp_t(vv, -A.u_max, A.u_max);
+    }
+  };
   auto load = [&](int r) {
     const uint32_t o = roff(r);
-    return Row<T>{ld(xh, o), ld(xu, o), ld(xv, o), ld(zb, o)};
+    Row<T> w = {ld(xh, o), ld(xu, o), ld(xv, o), ld(zb, o)};
+    // always in the convex second stage; a derivative evaluation of that y1
+    // says so at run time
+    if constexpr (CONVEX && BASE) complete(w.h, w.u, w.v);
+    else if constexpr (sizeof(T) == 4 && KOUT) { if (A.adj_in) complete(w.h, w.u, w.v); }
+    return w;
   };
   // a difference along x / y as the slope quantity (see QUICK)
   auto qx = [&](T d) -> T {
@@ -399,14 +427,25 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     T n;
     T bh[NB], bu[NB], bv[NB];
     T lo;  // FP32 form: low-order depth of the final row's base
+    T ru, rv;  // convex second stage: y1's velocities as stored (before E8/E9)
     // FP32 form: per-cell boundary endpoints (discharge, head)
     T q0, q1, hb0, hb1;
     // FP32 form with the further source terms: their per-cell fields
     T ndp, dsh, ddp, fx, fy, irate, icap, mu, sb0, sb1;
   };
-  T* const hl_in = (sizeof(T) == 4 && !KOUT) ? (A.stage == 0 ? (cur ? A.Uhl[1] : A.Uhl[0])
-                                                              : (cur ? A.Uhl[0] : A.Uhl[1]))
-                                             : nullptr;
+  // low-order depth read by this stage: U0's (first stage; convex second
+  // stage) or the final row's partial (later stages); the convex first stage
+  // reads none
+  T* const hl_in = (sizeof(T) == 4 && !KOUT)
+                       ? ((A.stage == 0 || CONVEX) ? (cur ? A.Uhl[1] : A.Uhl[0])
+                                                   : (cur ? A.Uhl[0] : A.Uhl[1]))
+                       : nullptr;
+  constexpr bool kLoIn = sizeof(T) == 4 && !KOUT && !(CONVEX && !BASE);
+  // convex second stage: the bases are U0 (the new state is written to the
+  // other buffer)
+  const T* const u0h = cur ? A.Uh[1] : A.Uh[0];
+  const T* const u0u = cur ? A.Uu[1] : A.Uu[0];
+  const T* const u0v = cur ? A.Uv[1] : A.Uv[0];
   // FP32: the per-cell boundary endpoints are loaded with the other cell-pass
   // inputs, ahead of the row prefetch and without a branch, so that no wait
   // in the cell pass also waits for the prefetch.  A field that is off reads
@@ -415,7 +454,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   // used.
   constexpr bool kPreBnd = sizeof(T) == 4 && BND;
   constexpr bool kBnd = BND;  // per-cell endpoint fields possible
-  const T* const fb = (sizeof(T) == 4 && !KOUT) ? hl_in : zb;
+  const T* const fb = kLoIn ? hl_in : zb;
   const T* const q0p = A.discharge == 2 ? A.q0f : fb;
   const T* const q1p = A.discharge == 2 ? A.q1f : fb;
   const T* const hb0p = A.head == 2 ? A.hb0f : fb;
@@ -439,7 +478,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     if constexpr (sizeof(T) == 4) e.n = ld(nf, o);
     else e.n = has_n ? ld(nf, o) : A.n_sh;
     e.lo = T(0);
-    if constexpr (sizeof(T) == 4 && !KOUT) e.lo = ld(hl_in, o);
+    if constexpr (kLoIn) e.lo = ld(hl_in, o);
+    e.ru = e.rv = T(0);
+    if constexpr (CONVEX && BASE) { e.ru = ld(xu, o); e.rv = ld(xv, o); }
     if constexpr (kPreBnd) {
       e.q0 = ld(q0p, o); e.q1 = ld(q1p, o); e.hb0 = ld(hb0p, o); e.hb1 = ld(hb1p, o);
     } else {
@@ -457,7 +498,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     if constexpr (base_ld) {
 #pragma unroll
       for (int j = 0; j < NACC; ++j) {
-        e.bh[j] = ld(oh_[j], o); e.bu[j] = ld(ou_[j], o); e.bv[j] = ld(ov_[j], o);
+        if constexpr (CONVEX) {
+          e.bh[j] = ld(u0h, o); e.bu[j] = ld(u0u, o); e.bv[j] = ld(u0v, o);
+        } else {
+          e.bh[j] = ld(oh_[j], o); e.bu[j] = ld(ou_[j], o); e.bv[j] = ld(ov_[j], o);
+        }
       }
     }
     return e;
@@ -742,30 +787,34 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
         // in every cell, the mix is the identity (a wave-uniform skip)
         bool may_mix = !has_n;
         if constexpr (EXT) may_mix = may_mix || A.mfull;
-        if (may_mix && nsh != ndp) {
+        if (may_mix) {
           T t = vdiv(h - dsh, ddp - dsh);
           t = clamp_t(t, T(0), T(1));
-          n = mad_t(ndp - nsh, t * t * (T(3) - T(2) * t), nsh);
+          n = (nsh != ndp) ? mad_t(ndp - nsh, t * t * (T(3) - T(2) * t), nsh) : nsh;
         }
+        // the friction acts on wet cells only: a wave without one skips it
         const bool wet = h > T(1e-6);
-        const T inv_h = vdiv(h, mad_t(h, h, T(1e-3)));
-        const T Sf = n * n * fsqrt_t(mad_t(u, u, v * v)) * pow43(inv_h);
-        T dum = T(-kG) * Sf * u;
-        T dvm = T(-kG) * Sf * v;
-        // overshoot cap with the full timestep (ManningRoughnessSourceTerm.cpp:74-81):
-        // pu = ((-g Sf) u) dt with Sf >= 0 (a product of n^2, a square root
-        // and a power; NaN fails every test), so where |pu| > |u| both are
-        // non-zero and pu's sign is the opposite of u's: the reference's sign
-        // test always passes there and is not repeated
-        const T pu = dum * dtT, pv = dvm * dtT;
-        if (fabs_t(pu) > fabs_t(u)) dum = div_dt(-u, dtT, rdt);
-        if (fabs_t(pv) > fabs_t(v)) dvm = div_dt(-v, dtT, rdt);
-        dudt = wet ? dudt + dum : dudt;
-        dvdt = wet ? dvdt + dvm : dvdt;
+        T Sf = T(0);
+        if (wet) {
+          const T inv_h = vdiv(h, mad_t(h, h, T(1e-3)));
+          Sf = n * n * fsqrt_t(mad_t(u, u, v * v)) * pow43(inv_h);
+          T dum = T(-kG) * Sf * u;
+          T dvm = T(-kG) * Sf * v;
+          // overshoot cap with the full timestep (ManningRoughnessSourceTerm.cpp:74-81):
+          // pu = ((-g Sf) u) dt with Sf >= 0 (a product of n^2, a square root
+          // and a power; NaN fails every test), so where |pu| > |u| both are
+          // non-zero and pu's sign is the opposite of u's: the reference's
+          // sign test always passes there and is not repeated
+          const T pu = dum * dtT, pv = dvm * dtT;
+          if (fabs_t(pu) > fabs_t(u)) dum = div_dt(-u, dtT, rdt);
+          if (fabs_t(pv) > fabs_t(v)) dvm = div_dt(-v, dtT, rdt);
+          dudt = dudt + dum;
+          dvdt = dvdt + dvm;
+        }
         if constexpr (KOUT) {
           if (A.dg_nh && own) {
             st(A.dg_nh, wo, n);
-            st(A.dg_sf, wo, wet ? Sf : T(0));
+            st(A.dg_sf, wo, Sf);  // 0 where dry
           }
         }
       }
@@ -946,7 +995,31 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           T bh = h, bu = u, bv = v;
           if constexpr (base_ld) { bh = X0.bh[j]; bu = X0.bu[j]; bv = X0.bv[j]; }
           T nh, nlo = T(0);
-          if (sizeof(T) == 4 && j == NACC - 1) {
+          T nu, nv;
+          if constexpr (CONVEX) {
+            if constexpr (BASE) {
+              // y2 = (y0 + y1)/2 + k1 dt/2: the halves of y0's high part
+              // and of y1 summed error-free, y0's low part, the increment
+              // and its rounding error added, then TwoSum into hi + lo
+              const T b0 = T(0.5) * bh, b1 = T(0.5) * h;
+              const T s2 = b0 + b1, sb = s2 - b0;
+              const T sl = (b0 - (s2 - sb)) + (b1 - sb);
+              const T p = dhdt * sc[j];
+              const T pe = fma_t(dhdt, sc[j], -p);
+              const T inc = ((pe + lo_in) + sl) + p;
+              nh = s2 + inc;
+              const T bb = nh - s2;
+              nlo = (s2 - (nh - bb)) + (inc - bb);
+              // y1's velocities as stored: y2 = y0 + k0 dt/2 + k1 dt/2
+              // whatever E8/E9 made of y1 as a stage input
+              nu = mad_t(dudt, sc[j], T(0.5) * (bu + X0.ru));
+              nv = mad_t(dvdt, sc[j], T(0.5) * (bv + X0.rv));
+            } else {  // y1 = y0 + k0 dt (a stage input: no low part)
+              nh = mad_t(dhdt, sc[j], bh);
+              nu = mad_t(dudt, sc[j], bu);
+              nv = mad_t(dvdt, sc[j], bv);
+            }
+          } else if (sizeof(T) == 4 && j == NACC - 1) {
             const T p = dhdt * sc[j];
             const T pe = fma_t(dhdt, sc[j], -p);   // exact: dhdt*sc == p + pe
             const T b = (pe + lo_in) + p;
@@ -956,25 +1029,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           } else {
             nh = mad_t(dhdt, sc[j], bh);
           }
-          T nu = mad_t(dudt, sc[j], bu);
-          T nv = mad_t(dvdt, sc[j], bv);
+          if constexpr (!CONVEX) {
+            nu = mad_t(dudt, sc[j], bu);
+            nv = mad_t(dvdt, sc[j], bv);
+          }
           if (j == 0) {  // row s+1 is complete after this stage
-            if (A.flags & 0x80) {  // E8
-              const bool dryc = nh <= A.h_dry;
-              nu = dryc ? T(0) : nu;
-              nv = dryc ? T(0) : nv;
-            }
-            if (A.flags & 0x100) {  // E9
-              nu = clamp_t(nu, -A.u_max, A.u_max);
-              nv = clamp_t(nv, -A.u_max, A.u_max);
-            }
+            // (the convex form stores y1 as it is; its readers complete it)
+            if constexpr (!(CONVEX && !BASE)) complete(nh, nu, nv);
             oh = nh; ou = nu; ov = nv; ol = nlo;
           }
           if (own) {
             st(oh_[j], wo, nh);
             st(ou_[j], wo, nu);
             st(ov_[j], wo, nv);
-            if constexpr (sizeof(T) == 4) {
+            if constexpr (sizeof(T) == 4 && !(CONVEX && !BASE)) {
               if (j == NACC - 1) st(cur ? A.Uhl[0] : A.Uhl[1], wo, nlo);
             }
           }
@@ -1042,7 +1110,8 @@ template <typename T, bool QUICK, bool NANBED, bool EXT>
 constexpr int stage_waves() {
   return sizeof(T) == 4 ? 4 : ((QUICK && !NANBED && !EXT) ? 3 : 2);
 }
-template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE, bool BND>
+template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE, bool BND,
+          bool CONVEX>
 __global__ __launch_bounds__(kWave * kWavesPerBlock)
 __attribute__((amdgpu_waves_per_eu(stage_waves<T, QUICK, NANBED, EXT>()))) void k_stage(const StageArgs<T> A) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -1050,7 +1119,7 @@ __attribute__((amdgpu_waves_per_eu(stage_waves<T, QUICK, NANBED, EXT>()))) void 
   const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
                                                   (int)(threadIdx.x >> 6));
   if (tile >= A.ntiles) return;
-  stage_tile<T, NACC, KOUT, QUICK, NANBED, EXT, BASE, BND>(A, tile, lane);
+  stage_tile<T, NACC, KOUT, QUICK, NANBED, EXT, BASE, BND, CONVEX>(A, tile, lane);
 }
 
 // Many catchments in one launch: a persistent grid walks a tile table of
@@ -1061,7 +1130,7 @@ __attribute__((amdgpu_waves_per_eu(stage_waves<T, QUICK, NANBED, EXT>()))) void 
 // does not depend on which other catchments share the launch.  The table
 // length is read from device memory so that captured graphs survive changes
 // of membership.
-template <typename T, int NACC, bool QUICK, bool GENERAL, bool BASE>
+template <typename T, int NACC, bool QUICK, bool GENERAL, bool BASE, bool CONVEX>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
     const StageArgs<T>* __restrict__ margs, const int2* __restrict__ tiles,
     const int* __restrict__ ntiles_p) {
@@ -1087,15 +1156,15 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
     if constexpr (GENERAL) {
       const bool ext = (a.eloss | a.infil | a.stageb | a.eddy) != 0;
       if (a.nanbed) {
-        stage_tile<T, NACC, false, false, true, true, BASE, true>(a, t, lane);
+        stage_tile<T, NACC, false, false, true, true, BASE, true, CONVEX>(a, t, lane);
         continue;
       }
       if (ext) {
-        stage_tile<T, NACC, false, QUICK, false, true, BASE, true>(a, t, lane);
+        stage_tile<T, NACC, false, QUICK, false, true, BASE, true, CONVEX>(a, t, lane);
         continue;
       }
     }
-    stage_tile<T, NACC, false, QUICK, false, false, BASE, GENERAL>(a, t, lane);
+    stage_tile<T, NACC, false, QUICK, false, false, BASE, GENERAL, CONVEX>(a, t, lane);
   }
 }
 
