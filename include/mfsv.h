@@ -173,7 +173,7 @@ int mfsv_set_params(mfsv_ctx* ctx, const mfsv_params* p);
  * row i < n = {c_i, a_i1..}, row n = {0, b_1..b_n}.  1 <= nstages <= 4. */
 int mfsv_set_scheme(mfsv_ctx* ctx, int nstages, const double* tableau);
 /* Work decomposition of the stage kernel: interior rows per wavefront tile
- * (1..256; 16 by default).  Results do not depend on it; the autotuner picks
+ * (1..64; 16 by default).  Results do not depend on it; the autotuner picks
  * it per grid shape.  No reference counterpart (SYCL chose its own ranges). */
 int mfsv_set_tile_rows(mfsv_ctx* ctx, int rows);
 int mfsv_get_tile_rows(const mfsv_ctx* ctx, int* rows);

@@ -756,7 +756,9 @@ int mfsv_set_scheme(mfsv_ctx* c, int nstages, const double* tab) {
 }
 
 int mfsv_set_tile_rows(mfsv_ctx* c, int rows) {
-  if (!c || rows < 1 || rows > 256) return MFSV_ERR_ARG;
+  if (!c) return MFSV_ERR_ARG;
+  // the storm's row factors of a tile come from a table of one row per lane
+  if (rows < 1 || rows > kWave) return fail(c, MFSV_ERR_ARG, "tile rows must be in 1..64");
   if (rows == c->rows) return MFSV_OK;
   drop_graph(c);
   c->rows = rows;

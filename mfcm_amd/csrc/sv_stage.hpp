@@ -712,19 +712,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       T ey0 = T(0), ey1 = T(0);
       if (storm) {
         const int k = r - y0;
-        if (k < kWave) {
-          if constexpr (kStormLds) {
-            ey0 = storm_rows[wv][0][k];
-            ey1 = storm_rows[wv][1][k];
-          } else {
-            ey0 = lane_value(eyA0, k);
-            ey1 = lane_value(eyA1, k);
-          }
+        // a tile has at most kWave rows (mfsv_set_tile_rows): row r's
+        // factors are always in the wave's table
+        if constexpr (kStormLds) {
+          ey0 = storm_rows[wv][0][k];
+          ey1 = storm_rows[wv][1][k];
         } else {
-          const Storm& st = A.storm;
-          const double yc = A.oy + (double(r) + 0.5) * A.dyd;
-          ey0 = storm_axis<T>(st, yc, st.cy0 + st.vy * gstep);
-          ey1 = storm_axis<T>(st, yc, st.cy0 + st.vy * (gstep + step_len));
+          ey0 = lane_value(eyA0, k);
+          ey1 = lane_value(eyA1, k);
         }
       }
       const uint32_t wo = lane_off + uint32_t(r) * row_bytes;
