@@ -217,10 +217,11 @@ def precision_study(sc0, horizon, rows=None, variants=("f32", "f64_zb_f32", "f64
     return out, res
 
 
-def _batched(scs, prec, rows, steps, warmup):
+def _batched(scs, prec, rows, steps, warmup, spinup_s=0.0):
     """Attempted timesteps of every catchment advanced together (one
-    persistent stage launch per RK stage, hipGraph chunks); accepted
-    cell-updates per second over the timed part."""
+    persistent stage launch per RK stage, hipGraph chunks), after an untimed
+    spin-up until every member has reached spinup_s simulated seconds;
+    accepted cell-updates per second over the timed part."""
     from mfcm_amd import mfsv
     from mfcm_amd.batch import Batch
     cats = []
@@ -231,6 +232,9 @@ def _batched(scs, prec, rows, steps, warmup):
         c.run_begin()
         b.add(c)
         cats.append(c)
+    while spinup_s > 0 and any(not st["done"] and st["time"] < spinup_s
+                               for st in (c.stats() for c in cats)):
+        b.run(512)
     b.run(warmup)
     device_sync()
     a0 = [c.stats()["accepted"] for c in cats]
@@ -246,14 +250,16 @@ def _batched(scs, prec, rows, steps, warmup):
     return cu / wall, wall
 
 
-def ensemble(n, steps, warmup, horizon_members, horizon):
+def ensemble(n, steps, warmup, horizon_members, horizon, spinup_s=300.0):
     """Batched ensemble: n catchments (sides log-uniform in [128, 1024], dx
     U[2, 10] m, rain U[10, 100] mm/h for U[15, 60] min, n U[0.02, 0.06], 1 h
     horizon each) advanced together by the persistent tile-table kernel with
     an autotuned tile height and graph-captured timesteps, in FP32 and in
     FP64, against the same catchments solved one after another by the plain
-    FP64 path.  Every leg times `steps` attempted timesteps per catchment
-    after `warmup` untimed ones; cell-updates count accepted timesteps only.
+    FP64 path.  Every leg spins each catchment up untimed from its dry start
+    to spinup_s simulated seconds, then times `steps` attempted timesteps per
+    catchment after `warmup` untimed ones; cell-updates count accepted
+    timesteps only.
     The FP32-batched over FP64-sequential ratio splits into a batching part
     (FP64 batched over FP64 sequential) and a precision part (FP32 batched
     over FP64 batched)."""
@@ -271,9 +277,9 @@ def ensemble(n, steps, warmup, horizon_members, horizon):
         timings[name] = {k: round(v, 1) for k, v in tuner.last_timings.items()}
         log(f"  {name} tile rows {rows[name]} (model {model_rows_batch(shapes, prec)}), "
             f"us/timestep {timings[name]}")
-    v32, wall32 = _batched(scs, mfsv.F32, rows["f32"], steps, warmup)
+    v32, wall32 = _batched(scs, mfsv.F32, rows["f32"], steps, warmup, spinup_s)
     log(f"  FP32 batched: {v32:.4g} cell-updates/s over {wall32:.2f} s")
-    v64b, wall64b = _batched(scs, mfsv.F64, rows["f64"], steps, warmup)
+    v64b, wall64b = _batched(scs, mfsv.F64, rows["f64"], steps, warmup, spinup_s)
     log(f"  FP64 batched: {v64b:.4g} cell-updates/s over {wall64b:.2f} s")
     wall64, cu64 = 0.0, 0
     for i, sc in enumerate(scs):
@@ -281,6 +287,7 @@ def ensemble(n, steps, warmup, horizon_members, horizon):
             log(f"  FP64 plain sequential: catchment {i}/{n}")
         c = mfsv.Catchment(sc, precision=mfsv.F64)
         c.run_begin()
+        spin_up(c, spinup_s)
         c.run(warmup, graph=True)
         device_sync()
         x = c.stats()["accepted"]
@@ -296,7 +303,7 @@ def ensemble(n, steps, warmup, horizon_members, horizon):
     out = {
         "catchments": n, "cells": cells, "tile_rows": rows,
         "tile_timings_us_per_timestep": timings,
-        "timed_timesteps_per_catchment": steps,
+        "timed_timesteps_per_catchment": steps, "spinup_s": spinup_s,
         "f32_batched": {"value": v32, "unit": "cell-updates/s", "wall_s": wall32,
                         "alg_GBs": v32 * B["f32"] / 1e9,
                         "hbm_frac": v32 * B["f32"] / 1e9 / HBM_PEAK_GBS},
