@@ -244,9 +244,12 @@ __device__ __forceinline__ Side<T> outflow_ghost(const Side<T>& c) {
 // (the FP32 form loads them with the row, FP64 where they are used).
 // CONVEX (FP32 form, Heun-2 only, NACC == 1): the stages in the convex form
 // y1 = y0 + k0 dt, y2 = (y0 + y1)/2 + k1 dt/2 (the same method; FP32 is not
-// held to the reference's summation order): the first stage writes y1 only,
-// the second reads y0 (with its low-order depth) and y1 and writes y2 with a
-// compensated depth, so no partial row y0 + k0 dt/2 is stored and read back.
+// held to the reference's summation order): the first stage writes y1 (its
+// depth rounded once from y0's high part, the low term of (y0 + y1)/2 that
+// this leaves out into the new state's low-order buffer), the second reads
+// y0, y1 and that low word and writes y2 with a compensated depth, so no
+// partial row y0 + k0 dt/2 is stored and read back and the FP64 volume of
+// hi + lo still tracks the accumulated rates.
 template <typename T, int NACC, bool KOUT, bool QUICK, bool NANBED, bool EXT, bool BASE, bool BND,
           bool CONVEX = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile, const int lane) {
@@ -433,14 +436,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     // FP32 form with the further source terms: their per-cell fields
     T ndp, dsh, ddp, fx, fy, irate, icap, mu, sb0, sb1;
   };
-  // low-order depth read by this stage: U0's (first stage; convex second
-  // stage) or the final row's partial (later stages); the convex first stage
-  // reads none
+  // low-order depth read by this stage: U0's (first stage) or what the
+  // previous stage left in the new state's low-order buffer (the final row's
+  // partial; in the convex form, the low term of (y0 + y1)/2)
   T* const hl_in = (sizeof(T) == 4 && !KOUT)
-                       ? ((A.stage == 0 || CONVEX) ? (cur ? A.Uhl[1] : A.Uhl[0])
-                                                   : (cur ? A.Uhl[0] : A.Uhl[1]))
+                       ? ((A.stage == 0) ? (cur ? A.Uhl[1] : A.Uhl[0])
+                                         : (cur ? A.Uhl[0] : A.Uhl[1]))
                        : nullptr;
-  constexpr bool kLoIn = sizeof(T) == 4 && !KOUT && !(CONVEX && !BASE);
+  constexpr bool kLoIn = sizeof(T) == 4 && !KOUT;
   // convex second stage: the bases are U0 (the new state is written to the
   // other buffer)
   const T* const u0h = cur ? A.Uh[1] : A.Uh[0];
@@ -993,9 +996,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
           T nu, nv;
           if constexpr (CONVEX) {
             if constexpr (BASE) {
-              // y2 = (y0 + y1)/2 + k1 dt/2: the halves of y0's high part
-              // and of y1 summed error-free, y0's low part, the increment
-              // and its rounding error added, then TwoSum into hi + lo
+              // y2 = (y0 + y1)/2 + k1 dt/2: the halves of the high parts of
+              // y0 and y1 summed error-free, the low term of (y0 + y1)/2
+              // that the first stage left (the input low word), the
+              // increment and its rounding error added, then TwoSum into
+              // hi + lo
               const T b0 = T(0.5) * bh, b1 = T(0.5) * h;
               const T s2 = b0 + b1, sb = s2 - b0;
               const T sl = (b0 - (s2 - sb)) + (b1 - sb);
@@ -1009,8 +1014,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
               // whatever E8/E9 made of y1 as a stage input
               nu = mad_t(dudt, sc[j], T(0.5) * (bu + X0.ru));
               nv = mad_t(dvdt, sc[j], T(0.5) * (bv + X0.rv));
-            } else {  // y1 = y0 + k0 dt (a stage input: no low part)
-              nh = mad_t(dhdt, sc[j], bh);
+            } else {
+              // y1 = hi(y0) + k0 dt rounded once, as the stencil of the
+              // second stage reads it; what that rounding and y0's low part
+              // leave out (y1 - y0 - k0 dt exactly, up to the rounding of
+              // this small sum) goes to the new state's low-order buffer as
+              // lo(y0) + (rounding)/2, the low term of (y0 + y1)/2
+              const T p = dhdt * sc[j];
+              const T pe = fma_t(dhdt, sc[j], -p);
+              nh = bh + p;
+              const T bb = nh - bh;
+              const T e = (bh - (nh - bb)) + (p - bb);  // hi(y0) + p == nh + e
+              nlo = fma_t(T(0.5), e + pe, lo_in);
               nu = mad_t(dudt, sc[j], bu);
               nv = mad_t(dvdt, sc[j], bv);
             }
@@ -1037,7 +1052,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
             st(oh_[j], wo, nh);
             st(ou_[j], wo, nu);
             st(ov_[j], wo, nv);
-            if constexpr (sizeof(T) == 4 && !(CONVEX && !BASE)) {
+            if constexpr (sizeof(T) == 4) {
               if (j == NACC - 1) st(cur ? A.Uhl[0] : A.Uhl[1], wo, nlo);
             }
           }

@@ -361,6 +361,13 @@ class SaintVenantSolver {
     check(mfsv_volume(ctx_->get(), &v), ctx_->get(), "mfsv_volume");
     return v;
   }
+  // The device controller's counters and FP64 volume integrals (the volume
+  // when the device run began and the integrated net source rate since).
+  mfsv_stats device_stats() const {
+    mfsv_stats s{};
+    check(mfsv_get_stats(ctx_->get(), &s), ctx_->get(), "mfsv_get_stats");
+    return s;
+  }
 
  private:
   void bind_and_upload(DataArray<double>& a, int field, const std::vector<double>& values) {

@@ -18,7 +18,9 @@
 //   --flags N            MFSV_E* extension bits (default 0)
 //   --quiet              no progress lines
 //   --dump-state DIR     also write the final h, u, v (raw float64) and
-//                        stats.txt to DIR, as the array mode below does
+//                        stats.txt to DIR, as the array mode below does (with
+//                        the device loop also volume0 and source_volume, the
+//                        terms of the FP64 mass balance)
 // and prints "solved: N timesteps, M repeated" at the end.
 //
 //   mfsv_drive <input-dir> <output-dir>
@@ -172,6 +174,11 @@ int run_config(GlobalConfig& gc, const RunOptions& ro, const std::string& dump) 
        << "repeated_timesteps = " << s.total_repeated_timesteps() << "\n"
        << "volume = " << model.solver().volume() << "\n"
        << "precision = " << (ro.precision == MFSV_F64 ? "f64" : "f32") << "\n";
+    if (model.solver().device_loop()) {  // the device controller's mass balance
+      const mfsv_stats ds = model.solver().device_stats();
+      st << "volume0 = " << ds.volume0 << "\n"
+         << "source_volume = " << ds.source_volume << "\n";
+    }
     if (!st) throw std::runtime_error("cannot write stats.txt");
   }
   std::cout << "solved: " << s.total_timesteps() << " timesteps, " << s.total_repeated_timesteps()
