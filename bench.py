@@ -244,10 +244,11 @@ def _batched(scs, prec, rows, steps, warmup, spinup_s=0.0):
     wall = time.perf_counter() - t0
     a1 = [c.stats()["accepted"] for c in cats]
     cu = sum((y - x) * sc.ncells for x, y, sc in zip(a0, a1, scs))
+    acc_frac = sum(y - x for x, y in zip(a0, a1)) / float(steps * len(cats))
     b.close()
     for c in cats:
         c.close()
-    return cu / wall, wall
+    return cu / wall, wall, acc_frac
 
 
 def ensemble(n, steps, warmup, horizon_members, horizon, spinup_s=300.0):
@@ -277,10 +278,10 @@ def ensemble(n, steps, warmup, horizon_members, horizon, spinup_s=300.0):
         timings[name] = {k: round(v, 1) for k, v in tuner.last_timings.items()}
         log(f"  {name} tile rows {rows[name]} (model {model_rows_batch(shapes, prec)}), "
             f"us/timestep {timings[name]}")
-    v32, wall32 = _batched(scs, mfsv.F32, rows["f32"], steps, warmup, spinup_s)
-    log(f"  FP32 batched: {v32:.4g} cell-updates/s over {wall32:.2f} s")
-    v64b, wall64b = _batched(scs, mfsv.F64, rows["f64"], steps, warmup, spinup_s)
-    log(f"  FP64 batched: {v64b:.4g} cell-updates/s over {wall64b:.2f} s")
+    v32, wall32, af32 = _batched(scs, mfsv.F32, rows["f32"], steps, warmup, spinup_s)
+    log(f"  FP32 batched: {v32:.4g} cell-updates/s over {wall32:.2f} s, accepted {af32:.3f}")
+    v64b, wall64b, af64 = _batched(scs, mfsv.F64, rows["f64"], steps, warmup, spinup_s)
+    log(f"  FP64 batched: {v64b:.4g} cell-updates/s over {wall64b:.2f} s, accepted {af64:.3f}")
     wall64, cu64 = 0.0, 0
     for i, sc in enumerate(scs):
         if i % 16 == 0:
@@ -305,9 +306,11 @@ def ensemble(n, steps, warmup, horizon_members, horizon, spinup_s=300.0):
         "tile_timings_us_per_timestep": timings,
         "timed_timesteps_per_catchment": steps, "spinup_s": spinup_s,
         "f32_batched": {"value": v32, "unit": "cell-updates/s", "wall_s": wall32,
+                        "accepted_fraction": af32,
                         "alg_GBs": v32 * B["f32"] / 1e9,
                         "hbm_frac": v32 * B["f32"] / 1e9 / HBM_PEAK_GBS},
         "f64_batched": {"value": v64b, "unit": "cell-updates/s", "wall_s": wall64b,
+                        "accepted_fraction": af64,
                         "alg_GBs": v64b * B["f64"] / 1e9,
                         "hbm_frac": v64b * B["f64"] / 1e9 / HBM_PEAK_GBS},
         "f64_plain_sequential": {"value": v64, "unit": "cell-updates/s", "wall_s": wall64},
