@@ -404,9 +404,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       vv = clamp_t(vv, -A.u_max, A.u_max);
     }
   };
-  auto load = [&](int r) {
+  // rawu, rawv: the row's velocities as stored, before the completion
+  // (the convex second stage combines y1's stored velocities; the compiler
+  // drops them elsewhere)
+  auto load = [&](int r, T& rawu, T& rawv) {
     const uint32_t o = roff(r);
     Row<T> w = {ld(xh, o), ld(xu, o), ld(xv, o), ld(zb, o)};
+    rawu = w.u;
+    rawv = w.v;
     // always in the convex second stage; a derivative evaluation of that y1
     // says so at run time
     if constexpr (CONVEX && BASE) complete(w.h, w.u, w.v);
@@ -423,14 +428,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     else return div_c(d, dy, rdy);
   };
 
-  Row<T> R0 = load(y0 - 2), R1 = load(y0 - 1), R2 = load(y0);
+  T r0u, r0v, r1u, r1v, r2u, r2v;  // stored velocities of rows R0, R1, R2
+  Row<T> R0 = load(y0 - 2, r0u, r0v), R1 = load(y0 - 1, r1u, r1v), R2 = load(y0, r2u, r2v);
   // Per-cell inputs of the cell pass (Manning n, accumulator bases of a
   // later stage).
   struct Extra {
     T n;
     T bh[NB], bu[NB], bv[NB];
     T lo;  // FP32 form: low-order depth of the final row's base
-    T ru, rv;  // convex second stage: y1's velocities as stored (before E8/E9)
     // FP32 form: per-cell boundary endpoints (discharge, head)
     T q0, q1, hb0, hb1;
     // FP32 form with the further source terms: their per-cell fields
@@ -482,8 +487,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     else e.n = has_n ? ld(nf, o) : A.n_sh;
     e.lo = T(0);
     if constexpr (kLoIn) e.lo = ld(hl_in, o);
-    e.ru = e.rv = T(0);
-    if constexpr (CONVEX && BASE) { e.ru = ld(xu, o); e.rv = ld(xv, o); }
     if constexpr (kPreBnd) {
       e.q0 = ld(q0p, o); e.q1 = ld(q1p, o); e.hb0 = ld(hb0p, o); e.hb1 = ld(hb1p, o);
     } else {
@@ -535,7 +538,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
     // ahead (its occupancy is lower, so memory latency needs the longer lead)
     Extra X0;
     if constexpr (sizeof(T) == 4) X0 = load_extra(c - 1 < y0 ? y0 : c - 1);
-    const Row<T> Rn = load(c + 2 > y1 + 1 ? y1 + 1 : c + 2);  // prefetch
+    T rnu, rnv;
+    const Row<T> Rn = load(c + 2 > y1 + 1 ? y1 + 1 : c + 2, rnu, rnv);  // prefetch
     if constexpr (sizeof(T) == 8) {
       X0 = Xn;
       Xn = load_extra(c < y0 ? y0 : c);
@@ -1012,8 +1016,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
               nlo = (s2 - (nh - bb)) + (inc - bb);
               // y1's velocities as stored: y2 = y0 + k0 dt/2 + k1 dt/2
               // whatever E8/E9 made of y1 as a stage input
-              nu = mad_t(dudt, sc[j], T(0.5) * (bu + X0.ru));
-              nv = mad_t(dvdt, sc[j], T(0.5) * (bv + X0.rv));
+              nu = mad_t(dudt, sc[j], T(0.5) * (bu + r0u));
+              nv = mad_t(dvdt, sc[j], T(0.5) * (bv + r0v));
             } else {
               // y1 = hi(y0) + k0 dt rounded once, as the stencil of the
               // second stage reads it; what that rounding and y0's low part
@@ -1068,6 +1072,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       }
     }
     R0 = R1; R1 = R2; R2 = Rn;
+    r0u = r1u; r1u = r2u; r2u = rnu;
+    r0v = r1v; r1v = r2v; r2v = rnv;
     if constexpr (!(QUICK && sizeof(T) == 8)) { qsh = qnh; qsu = qnu; qsv = qnv; qsz = qnz; }
     if constexpr (EXT) { pp_uy = p_uy; pp_vy = p_vy; }
     if constexpr (EXT || NANBED) { p_ux = sux; p_vx = svx; }
