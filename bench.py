@@ -86,7 +86,7 @@ def time_precision(sc, prec, steps, warmup, stage_samples, rows=0, spinup_s=0.0,
     timings = None
     if not rows:
         tuner = TileTuner()
-        rows = tuner.tune(sc, prec)
+        rows = tuner.tune(sc, prec, spinup_s=spinup_s)
         timings = {k: round(v, 1) for k, v in tuner.last_timings.items()}
         log(f"  tile rows {rows}: stage us {timings}")
     cat = mfsv.Catchment(sc, precision=prec)
@@ -274,7 +274,7 @@ def ensemble(n, steps, warmup, horizon_members, horizon, spinup_s=300.0):
     rows, timings = {}, {}
     for name, prec in (("f32", mfsv.F32), ("f64", mfsv.F64)):
         tuner = TileTuner()
-        rows[name] = tuner.tune_batch(scs, prec)
+        rows[name] = tuner.tune_batch(scs, prec, spinup_s=spinup_s)
         timings[name] = {k: round(v, 1) for k, v in tuner.last_timings.items()}
         log(f"  {name} tile rows {rows[name]} (model {model_rows_batch(shapes, prec)}), "
             f"us/timestep {timings[name]}")

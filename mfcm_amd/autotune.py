@@ -85,23 +85,31 @@ class TileTuner:
         return self.table.get(self.key(precision, nx, ny), model_rows(nx, ny, precision))
 
     def tune(self, scenario, precision, warmup=20, samples=20, candidates=CANDIDATES,
-             repeats=REPEATS) -> int:
+             repeats=REPEATS, spinup_s=0.0) -> int:
         """Time every candidate on a scratch catchment of the same scenario:
-        the state after the warmup is restored before each timing, the
-        candidates are timed in `repeats` interleaved rounds, and each one's
-        median stage-launch time decides."""
+        run untimed to spinup_s simulated seconds (the flow the caller will
+        time: a dry start's thin films make faces and friction cheaper than
+        an established flow's), then the state after the warmup is restored
+        before each timing, the candidates are timed in `repeats` interleaved
+        rounds, and each one's median stage-launch time decides."""
         import statistics
 
         from .mfsv import Catchment
         sc = copy.copy(scenario)
+        # the restored state does not rewind the clock: a horizon the tuning
+        # cannot reach keeps the run from finishing between candidates
+        sc.t_end = 4.0 * max(float(sc.t_end), float(spinup_s))
         cat = Catchment(sc, precision=precision)
         try:
             cat.run_begin()
+            st = cat.stats()
+            while st["time"] < spinup_s and not st["done"]:
+                st = cat.run(512, graph=True)
             cat.run(warmup, graph=True)
             snap = cat.state()
             times = {r: [] for r in candidates}
-            for _ in range(repeats):
-                for r in candidates:
+            for k in range(repeats):
+                for r in (candidates if k % 2 == 0 else tuple(reversed(candidates))):
                     cat.set_state(*snap)
                     cat.set_tile_rows(r)
                     cat.run(samples, graph=2)
@@ -121,27 +129,40 @@ class TileTuner:
         key = f"{int(precision)}:batch{len(shapes)}:{sum(nx * ny for nx, ny in shapes)}"
         return self.table.get(key, model_rows_batch(shapes, precision))
 
-    def tune_batch(self, scenarios, precision, warmup=16, samples=48,
-                   candidates=CANDIDATES, repeats=REPEATS) -> int:
+    def tune_batch(self, scenarios, precision, warmup=16, samples=64,
+                   candidates=CANDIDATES, repeats=REPEATS, spinup_s=0.0) -> int:
         """Time a shared tile height for a batch: scratch copies of the members
-        run `samples` timesteps per candidate (the batch's event-timed run),
+        run untimed until every one has reached spinup_s simulated seconds,
+        then `samples` timesteps per candidate (the batch's event-timed run),
         each timing from the members' states after the warmup, in `repeats`
-        interleaved rounds; the median time per timestep decides."""
+        rounds that alternate their order (no candidate always runs first or
+        last); the median time per timestep decides."""
         import statistics
 
         from .batch import Batch
         from .mfsv import Catchment
-        cats = [Catchment(copy.copy(sc), precision=precision) for sc in scenarios]
+        # the restored states do not rewind the members' clocks: horizons the
+        # tuning cannot reach keep members from finishing (and leaving the
+        # batch) between candidates
+        scratch = []
+        for sc in scenarios:
+            sc = copy.copy(sc)
+            sc.t_end = 4.0 * max(float(sc.t_end), float(spinup_s))
+            scratch.append(sc)
+        cats = [Catchment(sc, precision=precision) for sc in scratch]
         b = Batch(precision)
         times = {r: [] for r in candidates}
         try:
             for c in cats:
                 c.run_begin()
                 b.add(c)
+            while spinup_s > 0 and any(not st["done"] and st["time"] < spinup_s
+                                       for st in (c.stats() for c in cats)):
+                b.run(512)
             b.run(warmup)
             snaps = [c.state() for c in cats]
-            for _ in range(repeats):
-                for r in candidates:
+            for k in range(repeats):
+                for r in (candidates if k % 2 == 0 else tuple(reversed(candidates))):
                     for c, sn in zip(cats, snaps):
                         c.set_state(*sn)
                         c.set_tile_rows(r)
