@@ -354,10 +354,17 @@ template <typename T>
 __device__ __forceinline__ T minmod_c(T ql, T qr, T qc) {
   return minmod2(T(2) * minmod2(ql, qr), T(0.5) * qc);
 }
+// The one-sided and isolated cases given the two-sided value `in`: the
+// remaining quotient at an edge, zero without neighbours.  HALF: the halved
+// form of minmod_ch (the remaining quotient halved, exactly).
+template <bool HALF, typename T>
+__device__ __forceinline__ T minmod_edge(T in, T ql, T qr, bool hasl, bool hasr) {
+  const T l = HALF ? T(0.5) * ql : ql, r = HALF ? T(0.5) * qr : qr;
+  return hasl ? (hasr ? in : l) : (hasr ? r : T(0));
+}
 template <typename T>
 __device__ __forceinline__ T minmod_q(T ql, T qr, T qc, bool hasl, bool hasr) {
-  const T in = minmod_c(ql, qr, qc);
-  return hasl ? (hasr ? in : ql) : (hasr ? qr : T(0));
+  return minmod_edge<false>(minmod_c(ql, qr, qc), ql, qr, hasl, hasr);
 }
 // Half of minmod_c / minmod_q, exactly: med3(med3(ql, qr, 0), qc/4, 0)
 // selects the same operand at half scale.
@@ -367,8 +374,7 @@ __device__ __forceinline__ T minmod_ch(T ql, T qr, T qc) {
 }
 template <typename T>
 __device__ __forceinline__ T minmod_qh(T ql, T qr, T qc, bool hasl, bool hasr) {
-  const T in = minmod_ch(ql, qr, qc);
-  return hasl ? (hasr ? in : T(0.5) * ql) : (hasr ? T(0.5) * qr : T(0));
+  return minmod_edge<true>(minmod_ch(ql, qr, qc), ql, qr, hasl, hasr);
 }
 
 // The Rusanov speed |un + sign(un) c| of FluxKernel.cpp:225-226 with c >= 0:
