@@ -18,6 +18,8 @@ SRC = os.path.join(HERE, "csrc", "mfsv.hip")
 OUT = os.path.join(HERE, "libmfsv_hip.so")
 DRIVER_SRC = os.path.join(HERE, "host", "mfsv_drive.cpp")
 DRIVER = os.path.join(HERE, "host", "mfsv_drive")
+PROBE_SRC = os.path.join(HERE, "csrc", "probe_march.hip")
+PROBE = os.path.join(HERE, "probe_march")
 ARCH = os.environ.get("MFSV_ARCH", "gfx950")
 
 FLAGS = [
@@ -47,7 +49,24 @@ def build(verbose: bool = False, save_asm: bool = False) -> str:
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                         "--cuda-device-only", "-S", "-o", asm, SRC], check=True)
     build_driver(verbose)
+    build_probe(verbose)
     return OUT
+
+
+def build_probe(verbose: bool = False) -> str:
+    """The memory-pattern probe (a measurement aid beside the library: the
+    stage kernels' access pattern without their arithmetic)."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    cmd = [hipcc, "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-o", PROBE + ".tmp", PROBE_SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building the memory-pattern probe failed")
+    os.replace(PROBE + ".tmp", PROBE)
+    return PROBE
 
 
 def build_driver(verbose: bool = False) -> str:

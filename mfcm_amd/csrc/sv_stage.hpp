@@ -584,8 +584,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       sux = mm_c(qwu, qeu, qcu);
       svx = mm_c(qwv, qev, qcv);
       szx = mm_c(qwz, qez, qcz);
-      // (FP64 selects on every strip: a branch there costs it registers)
-      if (sizeof(T) == 8 || !xint) {
+      if (!xint) {
         shx = mm_e(shx, qwh, qeh, hasW, hasE);
         sux = mm_e(sux, qwu, qeu, hasW, hasE);
         svx = mm_e(svx, qwv, qev, hasW, hasE);
@@ -604,7 +603,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
       T yu = mm_c(qsu, qnu, qy(R2.u - R0.u));
       T yv = mm_c(qsv, qnv, qy(R2.v - R0.v));
       T yz = mm_c(qsz, qnz, qy(R2.z - R0.z));
-      if (sizeof(T) == 8 || !(hasS && hasN)) {  // wave-uniform: the grid's first or last row
+      if (!(hasS && hasN)) {  // wave-uniform: the grid's first or last row
         yh = mm_e(yh, qsh, qnh, hasS, hasN);
         yu = mm_e(yu, qsu, qnu, hasS, hasN);
         yv = mm_e(yv, qsv, qnv, hasS, hasN);
