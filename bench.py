@@ -88,7 +88,7 @@ def time_precision(sc, prec, steps, warmup, stage_samples, rows=0, spinup_s=0.0,
         tuner = TileTuner()
         rows = tuner.tune(sc, prec, spinup_s=spinup_s)
         timings = {k: round(v, 1) for k, v in tuner.last_timings.items()}
-        log(f"  tile rows {rows}: stage us {timings}")
+        log(f"  tile rows {rows}: us per timestep {timings}")
     cat = mfsv.Catchment(sc, precision=prec)
     if rows:
         cat.set_tile_rows(rows)
@@ -126,7 +126,7 @@ def time_precision(sc, prec, steps, warmup, stage_samples, rows=0, spinup_s=0.0,
         "stage_ms_mean": st_ms / max(launches, 1), "stage_launches": launches,
         "sim_time_s": s1["time"], "done": s2["done"],
         "device_bytes": cat.device_bytes(),
-        "tile_rows": cat.tile_rows(), "tile_timings_us": timings,
+        "tile_rows": cat.tile_rows(), "tile_timings_us_per_timestep": timings,
     }
     cat.close()
     return out, state
@@ -455,6 +455,39 @@ def valu_issue(name, size, stage_us):
             "clock_ghz": ENGINE_CLOCK_HZ / 1e9, "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
 
 
+PROBE = os.path.join(ROOT, "mfcm_amd", "probe_march")
+
+
+def memory_floor(size, rows32, rows64):
+    """The stage kernels' access pattern without their arithmetic
+    (mfcm_amd/probe_march, built beside the library): launch times of
+    memory-only kernels that move exactly one stage launch's bytes the same
+    way, at the tile heights timed here, plus a device copy and a read-only
+    stream.  None when the probe is not built."""
+    import subprocess
+    if not os.path.exists(PROBE):
+        return None
+    try:
+        out = subprocess.run([PROBE, str(size), str(size), str(rows32), str(rows64)],
+                             capture_output=True, text=True, timeout=120)
+    except (OSError, subprocess.TimeoutExpired) as e:
+        log(f"memory-pattern probe failed: {e}")
+        return None
+    if out.returncode != 0:
+        log(f"memory-pattern probe failed: {out.stderr.strip()[-300:]}")
+        return None
+    vals = {}
+    for line in out.stdout.splitlines():
+        if "|" not in line:
+            continue
+        tok = line.split("|", 1)[1].split()
+        if len(tok) == 2:
+            vals[tok[0]] = float(tok[1])
+        elif len(tok) == 5 and tok[1] == "rows" and tok[3] == "us":
+            vals[tok[0]] = float(tok[4])
+    return vals
+
+
 def run_to_end(sc, prec, rows=0):
     from mfcm_amd import mfsv
     cat = mfsv.Catchment(sc, precision=prec)
@@ -750,6 +783,8 @@ def main():
         cpu0 = cpu_config0(args.cpu_config0)
 
     B = {"f64": 8 * (15 + 2 * 2), "f32": 4 * (15 + 2 * 2)}
+    log("memory-pattern probe")
+    floor = memory_floor(args.size, res["f32"]["tile_rows"], res["f64"]["tile_rows"])
     roof = {}
     for name in ("f32", "f64"):
         r = res[name]
@@ -762,6 +797,16 @@ def main():
                       "stage_us_mean": r["stage_ms_mean"] * 1e3,
                       "bytes_per_cell_update": B[name],
                       "valu_issue": valu_issue(name, args.size, r["stage_ms_mean"] * 1e3)}
+        if floor and f"{name}_s1" in floor and f"{name}_s2" in floor:
+            # what this access pattern allows with free arithmetic
+            fl_us = 0.5 * (floor[f"{name}_s1"] + floor[f"{name}_s2"])
+            fl_ach = (B[name] / 2.0) * N / (fl_us * 1e-6) / 1e9
+            roof[name]["memory_floor"] = {
+                "stage_pattern_us": [floor[f"{name}_s1"], floor[f"{name}_s2"]],
+                "stage_us_mean": fl_us, "achieved": fl_ach, "frac": fl_ach / HBM_PEAK_GBS,
+                "kernel_over_floor": r["stage_ms_mean"] * 1e3 / fl_us,
+                "device_copy_GBs": floor.get("copy_GBs"), "read_only_GBs": floor.get("read_GBs"),
+                "tile_rows": r["tile_rows"], "source": os.path.relpath(PROBE, ROOT)}
     f32, f64 = res["f32"], res["f64"]
     line = {
         "metric": METRIC,

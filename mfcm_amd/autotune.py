@@ -84,14 +84,17 @@ class TileTuner:
     def rows_for(self, nx, ny, precision) -> int:
         return self.table.get(self.key(precision, nx, ny), model_rows(nx, ny, precision))
 
-    def tune(self, scenario, precision, warmup=20, samples=20, candidates=CANDIDATES,
+    def tune(self, scenario, precision, warmup=20, samples=64, candidates=CANDIDATES,
              repeats=REPEATS, spinup_s=0.0) -> int:
         """Time every candidate on a scratch catchment of the same scenario:
         run untimed to spinup_s simulated seconds (the flow the caller will
         time: a dry start's thin films make faces and friction cheaper than
         an established flow's), then the state after the warmup is restored
         before each timing, the candidates are timed in `repeats` interleaved
-        rounds, and each one's median stage-launch time decides."""
+        rounds, and each one's median time per timestep decides.  A timing is
+        the replay of `samples` timesteps from the timestep graph (the launches
+        the caller's run makes), after an untimed chunk at the new height that
+        captures that graph."""
         import statistics
 
         from .mfsv import Catchment
@@ -112,10 +115,9 @@ class TileTuner:
                 for r in (candidates if k % 2 == 0 else tuple(reversed(candidates))):
                     cat.set_state(*snap)
                     cat.set_tile_rows(r)
-                    cat.run(samples, graph=2)
-                    ms, n = cat.last_stage_ms()
-                    if n:
-                        times[r].append(ms / n * 1e3)
+                    cat.run(32, graph=True)
+                    cat.run(samples, graph=True)
+                    times[r].append(cat.last_run_ms() / samples * 1e3)
             timings = {r: statistics.median(v) for r, v in times.items() if v}
         finally:
             cat.close()
@@ -133,8 +135,9 @@ class TileTuner:
                    candidates=CANDIDATES, repeats=REPEATS, spinup_s=0.0) -> int:
         """Time a shared tile height for a batch: scratch copies of the members
         run untimed until every one has reached spinup_s simulated seconds,
-        then `samples` timesteps per candidate (the batch's event-timed run),
-        each timing from the members' states after the warmup, in `repeats`
+        then `samples` timesteps per candidate (the batch's event-timed run,
+        after two untimed ones at the new height), each timing from the
+        members' states after the warmup, in `repeats`
         rounds that alternate their order (no candidate always runs first or
         last); the median time per timestep decides."""
         import statistics
@@ -166,6 +169,7 @@ class TileTuner:
                     for c, sn in zip(cats, snaps):
                         c.set_state(*sn)
                         c.set_tile_rows(r)
+                    b.run(2)
                     b.run(samples)
                     times[r].append(b.last_run_ms() / samples * 1e3)
         finally:
