@@ -1,7 +1,8 @@
 """The tile height of the stage kernel is a work decomposition only: for every
 height the states are bit-identical and the accepted / rejected timestep
 counts equal (the FP64 volume is summed per tile first, so it agrees to
-rounding).  Heights below, at and above the grid's rows, with a ragged last
+rounding).  Heights from a single row (a march of three rows, odd in the
+FP32 form's two-row loop) up to above the grid's rows, with a ragged last
 tile, on the storm catchment (drifting storm, Voronoi roughness, physics
 flags) in both precisions."""
 import numpy as np
@@ -35,7 +36,7 @@ def test_tile_height_does_not_change_results(gpu_lib, prec_name):
     _say(prec_name, "storm catchment 130x130, rows 18 (reference)")
     state0, acc0, rej0, vol0 = _run(sc, prec, 18)
     assert acc0 > 0
-    for rows in (4, 6, 8, 13, 64):
+    for rows in (1, 2, 3, 4, 6, 8, 13, 64):
         _say(prec_name, "rows", rows)
         state, acc, rej, vol = _run(sc, prec, rows)
         assert (acc, rej) == (acc0, rej0), rows
