@@ -12,7 +12,10 @@
 // device's streaming limits beside it.
 //
 //   build: hipcc -O3 --offload-arch=gfx950 -o probe_march probe_march.hip
-//   run:   ./probe_march [nx ny]      (default 2048 2048)
+//   run:   ./probe_march [nx ny [rows_fp32 rows_fp64]]
+//          (default 2048 2048, tile heights 18 and 24 in both precisions)
+// Each result line ends with "key value" pairs for scripts: copy_GBs,
+// read_GBs, f32_s1_us / f32_s2_us / f64_s1_us / f64_s2_us (per rows).
 //
 // Every access stays inside the fields: rows and columns are clamped into
 // the grid, stores go to owned cells inside the grid only.
@@ -46,8 +49,9 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 
 // NR row fields (read rows y0-2 .. y1+1), NX per-cell fields read at the
-// owned rows, NW fields written at the owned cells.  ROWMAJOR: tiles ordered
-// row block by row block (the FP64 stage kernels), else strip by strip.
+// owned rows, NW fields written at the owned cells.  Tile order as the stage
+// kernels number them: ROWMAJOR (FP64) row block by row block, else (FP32)
+// groups of 4 adjacent strips marched down together.
 template <typename T, int NR, int NX, int NW, bool ROWMAJOR>
 __global__ __launch_bounds__(256) void k_march(Fields F, int nx, int ny, int rows, int nrb,
                                                int ntiles) {
@@ -55,8 +59,16 @@ __global__ __launch_bounds__(256) void k_march(Fields F, int nx, int ny, int row
   const int tile = xcd_block(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6);
   if (tile >= ntiles) return;
   const int nstrips = ntiles / nrb;
-  const int strip = ROWMAJOR ? tile % nstrips : tile / nrb;
-  const int rb = ROWMAJOR ? tile / nstrips : tile % nrb;
+  int strip, rb;
+  if (ROWMAJOR) {
+    rb = tile / nstrips;
+    strip = tile - rb * nstrips;
+  } else {
+    const int gsz = 4 * nrb, g = tile / gsz, rem = tile - g * gsz;
+    const int gw = (g + 1) * 4 <= nstrips ? 4 : nstrips - g * 4;
+    rb = rem / gw;
+    strip = g * 4 + (rem - rb * gw);
+  }
   const int x = strip * 60 - 2 + lane;
   const int xc = clampi(x, 0, nx - 1);
   const bool own = lane >= 2 && lane < 62 && x < nx;
@@ -117,13 +129,13 @@ double time_us(K kernel, dim3 grid, dim3 block, int iters, Args... args) {
 }
 
 template <typename T, int NR, int NX, int NW, bool ROWMAJOR>
-void march(const Fields& F, int nx, int ny, int rows, const char* name) {
+void march(const Fields& F, int nx, int ny, int rows, const char* name, const char* key) {
   const int nstrips = (nx + 59) / 60, nrb = (ny + rows - 1) / rows, ntiles = nstrips * nrb;
   const double us = time_us(k_march<T, NR, NX, NW, ROWMAJOR>, dim3((ntiles + 3) / 4), dim3(256), 50, F,
                             nx, ny, rows, nrb, ntiles);
   const double bytes = double(nx) * ny * sizeof(T) * (NR + NX + NW);
-  std::printf("%-44s rows %2d  %8.2f us  %6.1f MB of fields  %6.0f GB/s\n", name, rows, us, bytes / 1e6,
-              bytes / us / 1e3);
+  std::printf("%-44s rows %2d  %8.2f us  %6.1f MB of fields  %6.0f GB/s  | %s rows %d us %.3f\n", name,
+              rows, us, bytes / 1e6, bytes / us / 1e3, key, rows, us);
 }
 
 int main(int argc, char** argv) {
@@ -132,6 +144,15 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "grid %d x %d out of range\n", nx, ny);
     return 2;
   }
+  std::vector<int> rows32 = {18, 24}, rows64 = {18, 24};
+  if (argc > 4) {
+    rows32 = {std::atoi(argv[3])};
+    rows64 = {std::atoi(argv[4])};
+  }
+  for (int r : rows32)
+    if (r < 1 || r > 64) { std::fprintf(stderr, "tile rows %d out of range\n", r); return 2; }
+  for (int r : rows64)
+    if (r < 1 || r > 64) { std::fprintf(stderr, "tile rows %d out of range\n", r); return 2; }
   Fields F;
   for (int k = 0; k < kMaxFields; ++k) {
     CK(hipMalloc(&F.f[k], size_t(nx) * ny * 8));
@@ -147,8 +168,10 @@ int main(int argc, char** argv) {
     CK(hipMemset(a, 0, n * 16));
     const double cu = time_us(k_copy, dim3(8192), dim3(256), 20, (const float4*)a, b, n);
     const double ru = time_us(k_read, dim3(8192), dim3(256), 20, (const float4*)a, o, n);
-    std::printf("device copy 256 MiB (read + write)           %8.2f us  %6.0f GB/s\n", cu, 2.0 * n * 16 / cu / 1e3);
-    std::printf("read-only stream 256 MiB                     %8.2f us  %6.0f GB/s\n", ru, 1.0 * n * 16 / ru / 1e3);
+    std::printf("device copy 256 MiB (read + write)           %8.2f us  %6.0f GB/s  | copy_GBs %.1f\n", cu,
+                2.0 * n * 16 / cu / 1e3, 2.0 * n * 16 / cu / 1e3);
+    std::printf("read-only stream 256 MiB                     %8.2f us  %6.0f GB/s  | read_GBs %.1f\n", ru,
+                1.0 * n * 16 / ru / 1e3, 1.0 * n * 16 / ru / 1e3);
     CK(hipFree(a));
     CK(hipFree(b));
     CK(hipFree(o));
@@ -158,16 +181,16 @@ int main(int argc, char** argv) {
   // n and y0's low-order depth at the owned rows, writes y1 and a low word;
   // stage 2 reads y1 and the bed as rows, n, the low word and y0 at the
   // owned rows, writes y2 and its low word.
-  for (int rows : {18, 24}) {
-    march<float, 4, 2, 4, false>(F, nx, ny, rows, "FP32 stage 1 pattern (4 rows, 2 cell, 4 out)");
-    march<float, 4, 5, 4, false>(F, nx, ny, rows, "FP32 stage 2 pattern (4 rows, 5 cell, 4 out)");
+  for (int rows : rows32) {
+    march<float, 4, 2, 4, false>(F, nx, ny, rows, "FP32 stage 1 pattern (4 rows, 2 cell, 4 out)", "f32_s1");
+    march<float, 4, 5, 4, false>(F, nx, ny, rows, "FP32 stage 2 pattern (4 rows, 5 cell, 4 out)", "f32_s2");
   }
   // FP64 reference summation order: stage 1 reads y0 and the bed as rows and
   // n at the owned rows, writes y1 and the partial row; stage 2 reads y1 and
   // the bed as rows, n and the partial at the owned rows, writes y2.
-  for (int rows : {18, 24}) {
-    march<double, 4, 1, 6, true>(F, nx, ny, rows, "FP64 stage 1 pattern (4 rows, 1 cell, 6 out)");
-    march<double, 4, 4, 3, true>(F, nx, ny, rows, "FP64 stage 2 pattern (4 rows, 4 cell, 3 out)");
+  for (int rows : rows64) {
+    march<double, 4, 1, 6, true>(F, nx, ny, rows, "FP64 stage 1 pattern (4 rows, 1 cell, 6 out)", "f64_s1");
+    march<double, 4, 4, 3, true>(F, nx, ny, rows, "FP64 stage 2 pattern (4 rows, 4 cell, 3 out)", "f64_s2");
   }
   for (int k = 0; k < kMaxFields; ++k) CK(hipFree(F.f[k]));
   return 0;
