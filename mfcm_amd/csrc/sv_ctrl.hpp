@@ -21,6 +21,9 @@ constexpr int kCtrlThreads = 1024;
 constexpr int kCtrlWaves = kCtrlThreads / kWave;
 
 constexpr int kCtrlVals = 2 + kMaxStages;  // volume, control number, stage rates
+// the control block in 8-byte words, read by the first threads of the block
+constexpr int kCtrlWords = int(sizeof(Ctrl) / 8);
+static_assert(sizeof(Ctrl) % 8 == 0 && kCtrlWords <= kWave, "control block: whole words, one wave");
 
 // Volume and per-stage source-rate sums and the control-number maximum of the
 // tile partials, in one pass: thread t takes tiles t, t + 1024, ... in that
@@ -28,9 +31,16 @@ constexpr int kCtrlVals = 2 + kMaxStages;  // volume, control number, stage rate
 // latency-bound), then a butterfly inside each wave and a fixed-order pass
 // over the waves.  Deterministic for a given tile count.
 // out[0] = volume partial sum, out[1] = max control number, out[2 + s] =
-// stage s rate sum.
-__device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double* out) {
+// stage s rate sum.  The first kCtrlWords threads also copy the control block
+// into ctrl_sh: their loads go out with the partials' (one memory round trip
+// for the whole kernel, instead of a wait for the block before the partials'
+// loads are issued), and the copy is visible to every thread on return.
+__device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double* out,
+                                          unsigned long long* ctrl_sh) {
   constexpr int kU = 4;
+  const bool cword = threadIdx.x < kCtrlWords;
+  unsigned long long cw = 0ull;
+  if (cword) cw = reinterpret_cast<const unsigned long long*>(A.ctrl)[threadIdx.x];
   double acc[kCtrlVals];
 #pragma unroll
   for (int q = 0; q < kCtrlVals; ++q) acc[q] = 0.0;
@@ -55,21 +65,21 @@ __device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double*
       for (int s = 0; s < kMaxStages; ++s) acc[2 + s] += v[u][2 + s];
     }
   }
-  const int nv = 2 + A.nstages;  // values in use
+  // every value takes part (the rates of stages beyond nstages are zeros), so
+  // that each level's cross-lane reads are all issued before one wait
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
     for (int q = 0; q < kCtrlVals; ++q) {
-      if (q < nv) {
-        const double o = __shfl_xor(acc[q], off, kWave);
-        acc[q] = (q == 1) ? ((o > acc[q]) ? o : acc[q]) : acc[q] + o;
-      }
+      const double o = __shfl_xor(acc[q], off, kWave);
+      acc[q] = (q == 1) ? ((o > acc[q]) ? o : acc[q]) : acc[q] + o;
     }
   }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & (kWave - 1)) == 0)
 #pragma unroll
     for (int q = 0; q < kCtrlVals; ++q) sh[w * kCtrlVals + q] = acc[q];
+  if (cword) ctrl_sh[threadIdx.x] = cw;
   __syncthreads();
   // thread q < kCtrlVals folds value q over the waves in wave order, all
   // its LDS reads issued before the first addition
@@ -92,15 +102,15 @@ __device__ __forceinline__ void tile_sums(const CtrlArgs& A, double* sh, double*
 
 // Controller (RungeKutta.cpp:154-167) + accept (RungeKutta.hpp:428-431) +
 // landing rules and outer-step transitions (TemporalScheme.hpp:112-149).
-__device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
+__device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh, unsigned long long* ctrl_sh) {
   Ctrl* C = A.ctrl;
-  // thread 0 reads the control block while the partial sums load, works on
-  // its copy and writes it back once
-  Ctrl L;
-  if (threadIdx.x == 0) L = *C;
+  // the control block arrives with the partial sums; thread 0 works on a
+  // copy of it and writes it back once
   double sums[kCtrlVals];
-  tile_sums(A, sh, sums);
+  tile_sums(A, sh, sums, ctrl_sh);
   if (threadIdx.x != 0) return;
+  Ctrl L;
+  __builtin_memcpy(&L, ctrl_sh, sizeof(Ctrl));
   if (L.done) return;
   const double vol = sums[0] * A.area;
   double rate = 0.0;
@@ -160,7 +170,8 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A, double* sh) {
 
 __global__ __launch_bounds__(kCtrlThreads) void k_ctrl(const CtrlArgs A) {
   __shared__ double sh[(kCtrlWaves + 1) * kCtrlVals];
-  ctrl_body(A, sh);
+  __shared__ unsigned long long ctrl_sh[kCtrlWords];
+  ctrl_body(A, sh, ctrl_sh);
 }
 
 // One block per member of a batch; the member count is read from device
@@ -168,8 +179,9 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl(const CtrlArgs A) {
 __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_batch(const CtrlArgs* __restrict__ args,
                                                              const int* __restrict__ nmembers_p) {
   __shared__ double sh[(kCtrlWaves + 1) * kCtrlVals];
+  __shared__ unsigned long long ctrl_sh[kCtrlWords];
   if ((int)blockIdx.x >= *nmembers_p) return;
-  ctrl_body(args[blockIdx.x], sh);
+  ctrl_body(args[blockIdx.x], sh, ctrl_sh);
 }
 
 // Host-driven mode: set the attempt's t_local / dt (and the buffer flip of an
