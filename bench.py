@@ -99,7 +99,7 @@ def time_precision(sc, prec, steps, warmup, stage_samples, rows=0, spinup_s=0.0,
     device_sync()
     s0 = cat.stats()
     t0 = time.perf_counter()
-    cat.run(steps, graph=True)
+    cat.advance(steps, graph=True)
     device_sync()
     wall = time.perf_counter() - t0
     ev_ms = cat.last_run_ms()

@@ -1235,7 +1235,10 @@ int mfsv_run(mfsv_ctx* c, int64_t max_timesteps, int use_graph) {
       }
     }
     left -= n;
-    if ((rc = read_ctrl(c))) return rc;
+    // the done flag decides whether another chunk follows; after the last
+    // chunk the control block is read once below, with the closing event
+    if (left > 0 || timed)
+      if ((rc = read_ctrl(c))) return rc;
     if (timed) {
       for (int64_t k = 0; k < n; ++k) {
         float m = 0.f;
@@ -1246,12 +1249,14 @@ int mfsv_run(mfsv_ctx* c, int64_t max_timesteps, int use_graph) {
     }
     if (c->hctrl->done) break;
   }
+  // the control block's copy goes in the stream ahead of the closing event,
+  // so one wait covers both
+  HIPCHK(c, hipMemcpyAsync(c->hctrl, c->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
-  if ((rc = read_ctrl(c))) return rc;
   c->cur = c->hctrl->cur;
   return MFSV_OK;
 }

@@ -478,6 +478,12 @@ class Catchment:
         self._check(lib().mfsv_run(self._h, int(max_timesteps), mode))
         return self.stats()
 
+    def advance(self, max_timesteps, graph=True):
+        """run() without the statistics read back afterwards (one host wait
+        per call: the end of the work)."""
+        mode = int(graph) if not isinstance(graph, bool) else (1 if graph else 0)
+        self._check(lib().mfsv_run(self._h, int(max_timesteps), mode))
+
     def run_step(self, graph=True):
         """The current outer step's inner loop on the device."""
         mode = int(graph) if not isinstance(graph, bool) else (1 if graph else 0)
