@@ -29,7 +29,7 @@ struct mfsv_batch {
   bool quick = true;
   bool any_nanbed = false;         // a member needs the general stage path
   bool convex = false;             // FP32 Heun-2: the convex form (convex_rk)
-  int nblocks = 0;                 // persistent grid
+  int grid_tiles = 0;              // table length the stage launches' grid was sized for
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int64_t graph_steps = 0;
@@ -151,6 +151,12 @@ int upload_tables(mfsv_batch* b) {
   BHIPCHK(b, hipMemcpyAsync(b->counts.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, b->stream));
   BHIPCHK(b, hipStreamSynchronize(b->stream));
   b->ntiles = nt;
+  // the stage launches' grid follows the table: a captured timestep with a
+  // grid for another table length is captured again
+  if (nt != b->grid_tiles) {
+    bdrop_graph(b);
+    b->grid_tiles = nt;
+  }
   b->gens.clear();
   for (auto* c : b->active) b->gens.push_back(c->gen);
   b->dirty = false;
@@ -162,7 +168,12 @@ void launch_batch_stage(mfsv_batch* b, int s) {
   const int* cnt = static_cast<const int*>(b->counts.p);
   const StageArgs<T>* m = static_cast<const StageArgs<T>*>(b->margs[s].p);
   const int2* t = static_cast<const int2*>(b->tiles.p);
-  dim3 grid((unsigned)b->nblocks), block(kWave * kWavesPerBlock);
+  // a wave per table entry (the hardware hands the next entry to whichever
+  // SIMD frees up first; the tiles of uneven members do not leave a static
+  // round-robin's tail); a table that grew after a capture is still covered
+  // by the kernel's stride loop
+  dim3 grid((unsigned)std::max(1, (b->grid_tiles + kWavesPerBlock - 1) / kWavesPerBlock)),
+      block(kWave * kWavesPerBlock);
   const bool base = s > 0;
   if constexpr (sizeof(T) == 4) {
     if (b->convex) {  // FP32 Heun-2 in the convex form: one row per stage
@@ -231,18 +242,6 @@ int mfsv_batch_create(int precision, int device, mfsv_batch** out) {
     mfsv_batch_destroy(b);
     return MFSV_ERR_HIP;
   }
-  // persistent grid: as many blocks per CU as the batch stage kernel's
-  // register count lets the CU hold (the runtime's occupancy query on the
-  // Heun-2 first-stage variant; 2 blocks = 2 waves per SIMD for FP64, 4 for
-  // FP32 with the current register counts)
-  int per_cu = 0;
-  hipError_t oe = precision == MFSV_F64
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_stage_batch<double, 2, true, false, false, false>, kWave * kWavesPerBlock, 0)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_stage_batch<float, 1, true, false, false, true>, kWave * kWavesPerBlock, 0);
-  if (oe != hipSuccess || per_cu < 1) per_cu = precision == MFSV_F64 ? 2 : 4;
-  b->nblocks = prop.multiProcessorCount * per_cu;
   *out = b;
   return MFSV_OK;
 }

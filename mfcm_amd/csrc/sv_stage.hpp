@@ -1177,14 +1177,16 @@ __attribute__((amdgpu_waves_per_eu(stage_waves<T, QUICK, NANBED, EXT>()))) void 
   stage_tile<T, NACC, KOUT, QUICK, NANBED, EXT, BASE, BND, CONVEX>(A, tile, lane);
 }
 
-// Many catchments in one launch: a persistent grid walks a tile table of
-// (member, tile) pairs; member m's arguments are margs[m].  Each tile runs
+// Many catchments in one launch: the grid walks a tile table of (member,
+// tile) pairs, a wave per entry when the grid was sized for the table, more
+// with the stride loop when the table has grown since; member m's arguments
+// are margs[m].  Each tile runs
 // exactly the code of the single-catchment kernel for its member (with
 // GENERAL, the coded-out or further-sources variant where the member needs
 // it), so a catchment's result
 // does not depend on which other catchments share the launch.  The table
-// length is read from device memory so that captured graphs survive changes
-// of membership.
+// length is read from device memory, so a captured launch is correct for any
+// table (the stride loop covers one longer than its grid).
 template <typename T, int NACC, bool QUICK, bool GENERAL, bool BASE, bool CONVEX>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_stage_batch(
     const StageArgs<T>* __restrict__ margs, const int2* __restrict__ tiles,
