@@ -219,7 +219,7 @@ def precision_study(sc0, horizon, rows=None, variants=("f32", "f64_zb_f32", "f64
 
 def _batched(scs, prec, rows, steps, warmup, spinup_s=0.0):
     """Attempted timesteps of every catchment advanced together (one
-    persistent stage launch per RK stage, hipGraph chunks), after an untimed
+    stage launch per RK stage, hipGraph chunks), after an untimed
     spin-up until every member has reached spinup_s simulated seconds;
     accepted cell-updates per second over the timed part."""
     from mfcm_amd import mfsv
@@ -254,7 +254,7 @@ def _batched(scs, prec, rows, steps, warmup, spinup_s=0.0):
 def ensemble(n, steps, warmup, horizon_members, horizon, spinup_s=300.0):
     """Batched ensemble: n catchments (sides log-uniform in [128, 1024], dx
     U[2, 10] m, rain U[10, 100] mm/h for U[15, 60] min, n U[0.02, 0.06], 1 h
-    horizon each) advanced together by the persistent tile-table kernel with
+    horizon each) advanced together by the tile-table kernel with
     an autotuned tile height and graph-captured timesteps, in FP32 and in
     FP64, against the same catchments solved one after another by the plain
     FP64 path.  Every leg spins each catchment up untimed from its dry start

@@ -290,7 +290,7 @@ int mfsv_last_stage_ms(mfsv_ctx* ctx, double* ms, int64_t* launches);
 /* ---- batches: many catchments advanced per launch.  No reference
  * counterpart (the reference runs one mesh per process, mfcm.cpp:38-49); the
  * batch drives the same per-catchment TemporalScheme loop as mfsv_run for
- * every member, with one persistent stage launch per RK stage over the
+ * every member, with one stage launch per RK stage over the
  * concatenated tile tables and one controller launch for all members. ---- */
 typedef struct mfsv_batch mfsv_batch;
 int mfsv_batch_create(int precision, int device, mfsv_batch** out);

@@ -24,8 +24,7 @@ STRIP = 60
 SIMDS = 1024          # 256 CUs x 4 SIMDs on MI355X
 # Waves per SIMD of the stage kernels at their register counts (F64, F32):
 # the single-catchment FP64 kernel of power-of-two spacing is built for 3
-# (<= 168 VGPRs), the batch kernel holds 2 FP64 / 4 FP32 waves (the batch's
-# persistent grid asks the runtime for the same figure).
+# (<= 168 VGPRs), the batch kernel holds 2 FP64 / 4 FP32 waves.
 OCCUPANCY = {0: 3, 1: 4}
 OCCUPANCY_BATCH = {0: 2, 1: 4}
 REPEATS = 3           # interleaved timing rounds per candidate (median)
@@ -48,7 +47,7 @@ def model_rows(nx: int, ny: int, precision: int) -> int:
 
 
 def model_rows_batch(shapes, precision: int) -> int:
-    """One tile height for a batch of (nx, ny) grids sharing a persistent
+    """One tile height for a batch of (nx, ny) grids sharing a stage
     launch: the same trade-off over the concatenated tile table, where the
     halo overhead is weighted by each grid's cells and the last strip of a
     grid wastes the rows beyond ny."""

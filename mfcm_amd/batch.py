@@ -2,7 +2,7 @@
 
 A ``Batch`` references ``Catchment`` members that keep their own device
 fields and controllers; ``run`` advances every unfinished member by a number
-of attempted timesteps, one persistent stage launch per RK stage for all of
+of attempted timesteps, one stage launch per RK stage for all of
 them.  A member's result is the same alone (``Catchment.run``) or in a batch:
 every tile runs the same kernel code on that member's data only.
 """

@@ -107,7 +107,7 @@ class FloodServer:
             self.held_back.add(req.rid)   # arrived, ready, but no room yet
             return False
         cat = Catchment(sc, precision=self.precision, device=self.device)
-        # the members share one persistent launch: the tile height that suits
+        # the members share one stage launch: the tile height that suits
         # the resident set with the new member (fewer halo rows once the
         # batch fills the chip; results do not depend on it)
         shapes = [(c.nx, c.ny) for _, c, _, _, _ in self.active.values()] + [(sc.nx, sc.ny)]

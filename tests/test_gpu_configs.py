@@ -288,7 +288,7 @@ def test_breach_4096_properties(gpu_lib):
 
 def test_ensemble_64_members_equal_alone(gpu_lib):
     """The 64-catchment ensemble (sides log-uniform in [128, 1024], FP32 in
-    one persistent launch per RK stage, hipGraph chunks) for 64 timesteps:
+    one launch per RK stage, hipGraph chunks) for 64 timesteps:
     every member finite, and a sample of members bit-identical to the same
     64 timesteps run alone."""
     from mfcm_amd import mfsv
