@@ -92,6 +92,9 @@ class FloodServer:
         self.refused = []
         self.prepared = {}
         self.gpu_s = 0.0          # wall time inside batch launches
+        self.retile = True        # one tile height for the resident set, kept current
+        self._members_gen = 0     # bumped by every admission and retirement
+        self._tiled_gen = -1
 
     def _admit(self, req, now):
         fut = self.prepared.get(req.rid)
@@ -119,6 +122,7 @@ class FloodServer:
         self.peak_active = max(self.peak_active, len(self.active) + 1)
         self.prepared.pop(req.rid, None)
         self.active[req.rid] = (req, cat, need, now, sc.ncells)
+        self._members_gen += 1
         return True
 
     def _retire(self, now):
@@ -145,6 +149,22 @@ class FloodServer:
             self.used -= need
             del self.active[rid]
             self.done.append(req)
+            self._members_gen += 1
+
+    def _retile(self):
+        """One tile height for the whole resident set, chosen again whenever
+        the set changes: a member admitted next to few others takes short
+        tiles (more, smaller waves to fill the chip), which cost it halo rows
+        for the rest of its run once the batch has grown.  Results do not
+        depend on the tile height; the batch rebuilds its tile table once."""
+        if not self.retile or self._tiled_gen == self._members_gen or not self.active:
+            return
+        self._tiled_gen = self._members_gen
+        shapes = [(c.nx, c.ny) for _, c, _, _, _ in self.active.values()]
+        rows = model_rows_batch(shapes, self.precision)
+        for _, c, _, _, _ in self.active.values():
+            if c.tile_rows() != rows:
+                c.set_tile_rows(rows)
 
     def serve(self, requests, wall_limit=None, keep_fields=False, log=None):
         """Run the stream: requests arrive at their `arrival` offsets from
@@ -166,6 +186,7 @@ class FloodServer:
                 nxt += 1
             while queue and self._admit(queue[0], t):
                 queue.pop(0)
+            self._retile()
             if self.active:
                 tb = time.perf_counter()
                 self.batch.run(self.chunk)
