@@ -386,34 +386,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   const bool storm = (A.flags & 0x08) != 0;
   const T dtT = T(dt);
   const T rdt = sizeof(T) == 8 ? T(1.0 / dt) : T(1.0f / float(dt));
-  // E4 storm factors: this lane's column at the step's two endpoint times
-  // (registers), and the factors of rows y0 .. y0+63 (computed by lane k for
-  // row y0 + k, kept in this wave's LDS slot and read back row by row as a
-  // broadcast, so they hold no registers across the march).
-  // The FP32 form has registers to spare and keeps the row factors in lane
-  // registers, read with v_readlane.
-  constexpr bool kStormLds = sizeof(T) == 8;
-  __shared__ T storm_rows[kWavesPerBlock][2][kWave];
-  const int wv = (int)(threadIdx.x >> 6);
-  T ex0 = T(0), ex1 = T(0), eyA0 = T(0), eyA1 = T(0);
-  if (storm) {
-    const Storm& st = A.storm;
-    const double t1 = gstep + step_len;
-    const double xc = A.ox + (double(x) + 0.5) * A.dxd;
-    ex0 = storm_axis<T>(st, xc, st.cx0 + st.vx * gstep);
-    ex1 = storm_axis<T>(st, xc, st.cx0 + st.vx * t1);
-    const double yc = A.oy + (double(y0 + lane) + 0.5) * A.dyd;
-    eyA0 = storm_axis<T>(st, yc, st.cy0 + st.vy * gstep);
-    eyA1 = storm_axis<T>(st, yc, st.cy0 + st.vy * t1);
-    if constexpr (kStormLds) {
-      storm_rows[wv][0][lane] = eyA0;
-      storm_rows[wv][1][lane] = eyA1;
-      // the wave's own writes, visible to all its lanes before the reads
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-  }
 
   // E8 / E9 on a completed state's velocities (no velocity where h <= h_dry,
   // components within +-u_max)
@@ -542,6 +514,36 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   // iterations load row y0's (needed a row or two later, so the loads are not
   // wasted on HBM); likewise the last iteration's prefetch reloads row y1 + 1
   if constexpr (sizeof(T) == 8) Xn = load_extra(y0);
+  // (the storm factors are computed while the first rows' loads are in
+  // flight: the loads above come first in the program)
+  // E4 storm factors: this lane's column at the step's two endpoint times
+  // (registers), and the factors of rows y0 .. y0+63 (computed by lane k for
+  // row y0 + k, kept in this wave's LDS slot and read back row by row as a
+  // broadcast, so they hold no registers across the march).
+  // The FP32 form has registers to spare and keeps the row factors in lane
+  // registers, read with v_readlane.
+  constexpr bool kStormLds = sizeof(T) == 8;
+  __shared__ T storm_rows[kWavesPerBlock][2][kWave];
+  const int wv = (int)(threadIdx.x >> 6);
+  T ex0 = T(0), ex1 = T(0), eyA0 = T(0), eyA1 = T(0);
+  if (storm) {
+    const Storm& st = A.storm;
+    const double t1 = gstep + step_len;
+    const double xc = A.ox + (double(x) + 0.5) * A.dxd;
+    ex0 = storm_axis<T>(st, xc, st.cx0 + st.vx * gstep);
+    ex1 = storm_axis<T>(st, xc, st.cx0 + st.vx * t1);
+    const double yc = A.oy + (double(y0 + lane) + 0.5) * A.dyd;
+    eyA0 = storm_axis<T>(st, yc, st.cy0 + st.vy * gstep);
+    eyA1 = storm_axis<T>(st, yc, st.cy0 + st.vy * t1);
+    if constexpr (kStormLds) {
+      storm_rows[wv][0][lane] = eyA0;
+      storm_rows[wv][1][lane] = eyA1;
+      // the wave's own writes, visible to all its lanes before the reads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
   // south slope quantities of the current row (rows y0-2 and y0-1 first)
   T qsh = qy(R1.h - R0.h), qsu = qy(R1.u - R0.u), qsv = qy(R1.v - R0.v), qsz = qy(R1.z - R0.z);
   // slopes of row c-1 that the cell pass and the y-face still need
