@@ -259,11 +259,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   constexpr uint32_t S = sizeof(T);
   constexpr int NB = NACC > 0 ? NACC : 1;
   Ctrl* __restrict__ C = A.ctrl;
-  // a finished device-controlled run skips its remaining captured stages;
-  // a derivative evaluation (KOUT) is always an explicit request
-  if constexpr (!KOUT) {
-    if (C->done) return;
-  }
+  // a finished device-controlled run skips its remaining captured stages (a
+  // derivative evaluation, KOUT, is always an explicit request): the tile
+  // then marches no row and stores nothing.  No branch: the control block's
+  // fields are loaded together, not one round trip before a branch and
+  // another after it.
+  const bool finished = !KOUT && C->done != 0;
 
   const int nx = A.nx, ny = A.ny;
   // tile order.  FP64: row block by row block (horizontal neighbours come
@@ -293,7 +294,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   }
   const int x = strip * kStrip - kHalo + lane;
   const int y0 = rb * A.rows;
-  const int y1 = (y0 + A.rows < ny) ? y0 + A.rows : ny;
+  const int y1 = finished ? y0 - 3 : ((y0 + A.rows < ny) ? y0 + A.rows : ny);
   const bool xin = x >= 0 && x < nx;
   const bool own = lane >= kHalo && lane < kWave - kHalo && xin;
   const bool hasW = x > 0, hasE = x < nx - 1;
