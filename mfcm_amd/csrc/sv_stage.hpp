@@ -1129,14 +1129,38 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& A, const int tile
   // c = y1 + 1, which computes no face or cell of the tile and stores nothing
   // (its loads are clamped into the grid).  FP64 keeps one row per trip: its
   // register budget at 3 waves per SIMD has no room for the second row.
+  // Issue priority falls as the wave gets through its tile.  A SIMD's
+  // arbiter otherwise favours its oldest wave among equals: the waves of a
+  // launch that fits the chip in one round start together but finish in
+  // the order they were dispatched (on the 2048^2 storm catchment the FP32
+  // first stage's waves took 29.5, 31.9, 35.1 and 39.1 us by dispatch
+  // quartile), and the rest of the launch ran on fewer and fewer waves per
+  // SIMD.  A wave past half of its rows yields to those behind it (priority
+  // 3 -> 2 at 1/2, 1 at 3/4, 0 at 15/16), so a SIMD's waves finish nearer
+  // together: FP32 stages 46.4 -> 41.7 us, FP64 at 24-row tiles 114 -> 105
+  // us.  Results do not depend on it.
+  const int nsteps = y1 - y0 + 2;
+  const int q1 = nsteps / 2, q2 = (3 * nsteps) / 4, q3 = (15 * nsteps) / 16;
+  __builtin_amdgcn_s_setprio(3);
+  auto yield_by_progress = [&](const int c) {
+    const int k = c - (y0 - 1);
+    if (k >= q3) __builtin_amdgcn_s_setprio(0);
+    else if (k >= q2) __builtin_amdgcn_s_setprio(1);
+    else if (k >= q1) __builtin_amdgcn_s_setprio(2);
+  };
   if constexpr (sizeof(T) == 4) {
     for (int c = y0 - 1; c <= y1; c += 2) {
+      yield_by_progress(c);
       row_step(c);
       row_step(c + 1);
     }
   } else {
-    for (int c = y0 - 1; c <= y1; ++c) row_step(c);
+    for (int c = y0 - 1; c <= y1; ++c) {
+      yield_by_progress(c);
+      row_step(c);
+    }
   }
+  __builtin_amdgcn_s_setprio(0);
 
   // the lanes that own a cell, then wave reductions (fixed butterfly order ->
   // deterministic per tile)
